@@ -1,0 +1,149 @@
+/*
+ * lqmc_hip.h -- C ABI of the MI355X (gfx950) determinant-QMC sweep engine.
+ *
+ * The reference (KieDani/LatticeQMC) has no native API: its boundary is the
+ * Python class LatticeQMC (lqmc/lqmc.py:14-408).  Each entry point below
+ * replaces one piece of that class, cited per declaration.  The Python drop-in
+ * latticeqmc_amd.LatticeQMC binds these through ctypes.
+ *
+ * Conventions
+ *  - Every entry returns int: 0 on success, a negative LQMC_E* code on error.
+ *    Nothing throws or aborts across the ABI; lqmc_last_error() returns a
+ *    thread-local message for the last failing call on the calling thread.
+ *  - Host pointers are borrowed for the duration of the call and copied.
+ *  - A context owns all device workspaces (allocated by lqmc_create /
+ *    lqmc_set_chain); sweeps allocate nothing.  A context is bound to one
+ *    device and one HIP stream and is not thread-safe; separate contexts may
+ *    be driven from separate threads.
+ *  - Fields use the reference layout: int8 +-1, shape (N, L) row-major,
+ *    element [i*L + l] = config[i, l] (lqmc/configuration.py:89,123-124).
+ *  - Spin 0 = up (V = exp(-lambda*s)), spin 1 = down (lqmc/lqmc.py:149).
+ */
+#ifndef LQMC_HIP_H
+#define LQMC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lqmc_ctx lqmc_ctx;
+
+enum {
+  LQMC_OK = 0,
+  LQMC_EINVAL = -1,    /* bad argument */
+  LQMC_EHIP = -2,      /* HIP runtime error */
+  LQMC_ENOMEM = -3,    /* device allocation failed */
+  LQMC_ESTATE = -4,    /* call out of order (e.g. sweep before set_chain) */
+  LQMC_ENUMERIC = -5,  /* device flagged NaN/Inf, zero pivot or replay underrun */
+};
+
+enum { LQMC_MODE_DQMC = 0, LQMC_MODE_REFERENCE = 1 };
+enum { LQMC_FP64 = 0, LQMC_FP32 = 1 };
+enum { LQMC_RNG_REPLAY = 0, LQMC_RNG_PHILOX = 1 };
+
+/* Device-side error flag bits reported in lqmc_stats.flags. */
+enum {
+  LQMC_FLAG_NAN_RATIO = 1,      /* a Metropolis ratio was NaN/Inf */
+  LQMC_FLAG_ZERO_PIVOT = 2,     /* QR or LU met a zero pivot */
+  LQMC_FLAG_REPLAY_UNDERRUN = 4,/* a replay sweep ran past the uploaded uniforms */
+  LQMC_FLAG_NAN_G = 8,          /* a recompute found NaN/Inf in G */
+};
+
+typedef struct {
+  int32_t mode;        /* LQMC_MODE_DQMC (textbook, stabilised) or LQMC_MODE_REFERENCE */
+  int32_t precision;   /* LQMC_FP64 or LQMC_FP32: storage of G, wraps and flip updates */
+  int32_t delay;       /* 1: rank-1 update per accepted flip; k>1: submatrix blocks of k sites */
+  int32_t n_stab;      /* slices between FP64 QR/UDV recomputes (Mode D) */
+  int32_t rng;         /* LQMC_RNG_REPLAY or LQMC_RNG_PHILOX */
+  int32_t trace;       /* 1: keep per-attempt ratio/accept of the last sweep */
+  int32_t obs_capacity;/* per-chain ring of per-sweep observables (0 = 4096) */
+  int32_t flip_threads;/* threads per flip workgroup (0 = automatic) */
+} lqmc_options;
+
+typedef struct {
+  uint64_t attempted;   /* attempted flips since create */
+  uint64_t accepted;    /* accepted flips since create */
+  uint64_t sweeps;      /* completed sweeps */
+  uint64_t measured;    /* measured sweeps accumulated in the mean G */
+  uint64_t recomputes;  /* FP64 recomputes performed */
+  double max_err;       /* max |G_propagated - G_recomputed| since create/reset */
+  uint32_t flags;       /* LQMC_FLAG_* bits */
+  uint32_t pad;
+} lqmc_stats;
+
+/* Replaces LatticeQMC.__init__ (lqmc/lqmc.py:16-79) for n_chains independent
+ * chains of sizes N[c] sites x L[c] slices (the batched launch also replaces the
+ * chain pool of lqmc/multiprocessing.py:252-289). */
+int lqmc_create(lqmc_ctx** ctx, int device, int n_chains, const int32_t* n_sites,
+                const int32_t* time_steps, const lqmc_options* opt);
+
+/* Replaces set_beta (lqmc/lqmc.py:93-117) and the Configuration passed in at
+ * construction (lqmc/configuration.py:91-124) for chain c: E_K = expm(-dtau K),
+ * E_K^-1 = expm(+dtau K), lambda = arccosh(exp(U dtau/2)), the hopping matrix K
+ * (N x N, used by the kinetic-energy estimator), the initial field and the
+ * Philox seed of the chain. */
+int lqmc_set_chain(lqmc_ctx* ctx, int c, const double* expk, const double* expk_inv,
+                   const double* kmat, double lambda, const int8_t* field, uint64_t seed);
+
+/* Computes the initial Green's functions: Mode D the stabilised G(L-1)
+ * (what _update_step's first inversion provides, lqmc/lqmc.py:303-307). */
+int lqmc_init(lqmc_ctx* ctx);
+
+/* Uniforms for replay mode (the np.random.rand() draws of lqmc/lqmc.py:317):
+ * u[s*L*N + t*N + i] for sweeps s = 0..n_sweeps-1 starting at the chain's next
+ * sweep, slice l = L-1-t, site i. */
+int lqmc_set_uniforms(lqmc_ctx* ctx, int c, const double* u, int64_t n_sweeps);
+
+/* Runs n_sweeps sweeps of every chain (_update_step, lqmc/lqmc.py:301-347, and
+ * the loops warmup_loop/measure_loop 349-375). measure != 0 accumulates the
+ * end-of-sweep G (lqmc/lqmc.py:371-372) and per-sweep observables.  Asynchronous
+ * on the context stream; stream may be NULL (context stream) or a hipStream_t. */
+int lqmc_sweep(lqmc_ctx* ctx, int n_sweeps, int measure, void* stream);
+
+/* Waits for the context stream and checks the device error flags. */
+int lqmc_sync(lqmc_ctx* ctx);
+
+/* Enables (1) or disables (0) replaying whole sweeps from captured hipGraphs. */
+int lqmc_use_graph(lqmc_ctx* ctx, int on);
+
+/* Green's functions of chain c into out[2*N*N] (float64, [spin][i][j]).
+ * which = 0: current G; 1: mean of measured end-of-sweep G (the return value of
+ * measure_loop, lqmc/lqmc.py:375); 2: G of the last measured sweep. */
+int lqmc_get_green(lqmc_ctx* ctx, int c, double* out, int which);
+
+/* Same, into device memory (e.g. a torch tensor's data_ptr()), on the given stream. */
+int lqmc_get_green_dev(lqmc_ctx* ctx, int c, double* dev_out, int which, void* stream);
+
+/* Current field of chain c into out[N*L] in the reference layout. */
+int lqmc_get_field(lqmc_ctx* ctx, int c, int8_t* out);
+
+/* Counters and error flags of chain c (c = -1: summed/maxed over chains). */
+int lqmc_get_stats(lqmc_ctx* ctx, int c, lqmc_stats* out);
+int lqmc_reset_stats(lqmc_ctx* ctx);
+
+/* Per-attempt trace of the last sweep (trace option): ratio[L*N] and accept[L*N]
+ * in attempt order t*N + i (the values _debug logs, lqmc/lqmc.py:217-232). */
+int lqmc_get_trace(lqmc_ctx* ctx, int c, double* ratio, uint8_t* accept);
+
+/* Per-sweep observables of the measured sweeps of chain c, oldest first, up to
+ * max_rows rows of (density, double occupancy, kinetic energy per site);
+ * *rows receives the count. */
+int lqmc_get_observables(lqmc_ctx* ctx, int c, double* out, int max_rows, int* rows);
+
+/* Average device duration (ms) of the kernels of one kind over the sweeps run
+ * since lqmc_profile(ctx, kind) was called, measured with HIP events on the
+ * context stream (graphs are bypassed while profiling).  kind: 0 = flip,
+ * 1 = GEMM, 2 = QR, 3 = solve; kind = -1 stops profiling. */
+int lqmc_profile(lqmc_ctx* ctx, int kind);
+int lqmc_profile_result(lqmc_ctx* ctx, double* avg_ms, int64_t* launches, double* work_units);
+
+const char* lqmc_last_error(void);
+int lqmc_destroy(lqmc_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LQMC_HIP_H */

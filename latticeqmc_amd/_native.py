@@ -1,0 +1,91 @@
+"""ctypes binding of liblqmc_hip.so (the C ABI in include/lqmc_hip.h).
+
+The library is built in-tree by ``python -m latticeqmc_amd.build``.  There is
+no CPU fallback: importing the binding without the library raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblqmc_hip.so")
+
+MODE_DQMC, MODE_REFERENCE = 0, 1
+FP64, FP32 = 0, 1
+RNG_REPLAY, RNG_PHILOX = 0, 1
+
+ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "ESTATE", -5: "ENUMERIC"}
+FLAG_NAMES = {1: "NaN/Inf ratio", 2: "zero pivot", 4: "replay uniforms exhausted", 8: "NaN/Inf in G"}
+
+EXPORTS = (
+    "lqmc_create", "lqmc_set_chain", "lqmc_init", "lqmc_set_uniforms", "lqmc_sweep",
+    "lqmc_sync", "lqmc_use_graph", "lqmc_get_green", "lqmc_get_green_dev", "lqmc_get_field",
+    "lqmc_get_stats", "lqmc_reset_stats", "lqmc_get_trace", "lqmc_get_observables",
+    "lqmc_profile", "lqmc_profile_result", "lqmc_last_error", "lqmc_destroy",
+)
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int32), ("precision", ctypes.c_int32), ("delay", ctypes.c_int32),
+                ("n_stab", ctypes.c_int32), ("rng", ctypes.c_int32), ("trace", ctypes.c_int32),
+                ("obs_capacity", ctypes.c_int32), ("flip_threads", ctypes.c_int32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("attempted", ctypes.c_uint64), ("accepted", ctypes.c_uint64),
+                ("sweeps", ctypes.c_uint64), ("measured", ctypes.c_uint64),
+                ("recomputes", ctypes.c_uint64), ("max_err", ctypes.c_double),
+                ("flags", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+
+
+class LqmcError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path=None):
+    """Load the shared library (once) and declare the prototypes."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `python -m latticeqmc_amd.build` "
+                          "(the GPU sweep has no CPU fallback)")
+    lib = ctypes.CDLL(path)
+    P, I, D, V = ctypes.POINTER, ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+    i32p, f64p = P(ctypes.c_int32), P(ctypes.c_double)
+    sig = {
+        "lqmc_create": (I, [P(V), I, I, i32p, i32p, P(Options)]),
+        "lqmc_set_chain": (I, [V, I, f64p, f64p, f64p, D, P(ctypes.c_int8), ctypes.c_uint64]),
+        "lqmc_init": (I, [V]),
+        "lqmc_set_uniforms": (I, [V, I, f64p, ctypes.c_int64]),
+        "lqmc_sweep": (I, [V, I, I, V]),
+        "lqmc_sync": (I, [V]),
+        "lqmc_use_graph": (I, [V, I]),
+        "lqmc_get_green": (I, [V, I, f64p, I]),
+        "lqmc_get_green_dev": (I, [V, I, V, I, V]),
+        "lqmc_get_field": (I, [V, I, P(ctypes.c_int8)]),
+        "lqmc_get_stats": (I, [V, I, P(Stats)]),
+        "lqmc_reset_stats": (I, [V]),
+        "lqmc_get_trace": (I, [V, I, f64p, P(ctypes.c_uint8)]),
+        "lqmc_get_observables": (I, [V, I, f64p, I, P(I)]),
+        "lqmc_profile": (I, [V, I]),
+        "lqmc_profile_result": (I, [V, f64p, P(ctypes.c_int64), f64p]),
+        "lqmc_last_error": (ctypes.c_char_p, []),
+        "lqmc_destroy": (I, [V]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = _lib.lqmc_last_error().decode(errors="replace") if _lib else ""
+        raise LqmcError(f"{what} failed ({ERRORS.get(rc, rc)}): {msg}")
+    return rc

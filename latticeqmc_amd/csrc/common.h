@@ -1,0 +1,113 @@
+// Device-side descriptors shared by the engine (host) and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lqmc {
+
+// Static description of one Markov chain (read-only on the device).
+struct ChainDev {
+  int N, L;
+  int mode;            // 0 dqmc, 1 reference
+  int pad0;
+  double lam;
+  double e1m, e1p;     // exp(-lambda), exp(+lambda): V entries for sigma*s = +1 / -1
+  double e2p, e2m;     // exp(+2 lambda), exp(-2 lambda)
+  uint64_t seed;
+  int8_t* field;       // [L][N]
+  double* ev;          // [2][L][N] exp(-sigma*lambda*s)
+  double* evi;         // [2][L][N] 1/ev
+  void* G;             // [2][N][N] in the form's precision
+  double* Gacc;        // [2][N][N] sum of measured G
+  double* Glast;       // [2][N][N] last measured G
+  double* hop;         // [N][N] off-diagonal part of K
+  double* uni;         // replay uniforms [n_replay][L][N]
+  double* tr_ratio;    // [L][N] trace
+  uint8_t* tr_acc;     // [L][N] trace
+  double* obs;         // [obs_cap][3]
+  int obs_cap;
+  int pad1;
+};
+
+// Mutable per-chain counters (device).
+struct ChainState {
+  unsigned long long sweep;        // completed sweeps (global sweep index)
+  unsigned long long replay_base;  // sweep index of uni[0]
+  unsigned long long replay_count; // sweeps held in uni
+  unsigned long long attempted;
+  unsigned long long accepted;
+  unsigned long long measured;
+  unsigned long long recomputes;
+  unsigned long long max_err_bits; // non-negative double as ordered bits
+  unsigned int flags;
+  unsigned int pad;
+};
+
+// Diagonal-scale kinds applied to GEMM rows (rs), inner index (ks) or columns (cs).
+enum ScaleKind : int { SC_NONE = 0, SC_PLAIN = 1, SC_INV = 2, SC_INV_BIG = 3, SC_SMALL = 4 };
+
+// C = alpha * diag(rs) * op(A) * diag(ks) * op(B) * diag(cs) + beta * C
+struct GemmTask {
+  const void* A;
+  const void* B;
+  void* C;
+  const double* rs;
+  const double* ks;
+  const double* cs;
+  int M, N, K;
+  int lda, ldb, ldc;
+  int transA, transB;
+  int rs_kind, ks_kind, cs_kind;
+  int tiles_n;      // tiles along N
+  int tile_base;    // first tile index of this task in the launch
+  int pad;
+  double alpha, beta;
+};
+
+// Pivoted Householder QR: W = Q diag(D) T (W destroyed).
+struct QrTask {
+  double* W;
+  double* Q;
+  double* D;
+  double* T;
+  int N;
+  int chain;
+};
+
+// Solve [X | RHS] (N x 2N, ld 2N) in place; RHS built from U (transposed,
+// scaled by 1/max(D,1)) or the identity; optional copy of Y to Yout (ld N).
+struct SolveTask {
+  double* aug;
+  const double* U;   // null -> RHS = I
+  const double* D;
+  double* Yout;      // optional
+  int N;
+  int add_identity;  // X += I before solving (reference-mode M = I + P)
+  int chain;
+  int pad;
+};
+
+// Compare the propagated G (form precision) with the recomputed Gst (f64),
+// record the max deviation, then overwrite G by Gst.
+struct FinishTask {
+  void* G;
+  const double* Gst;
+  int N;
+  int chain;
+  int compare;
+  int pad;
+};
+
+// Arguments of the flip kernels for step t (slice l = L-1-t of each chain).
+struct FlipArgs {
+  const ChainDev* chains;
+  ChainState* st;
+  int t;
+  int measure;
+  int rng;
+  int trace;
+  int delay;
+  int pad;
+};
+
+}  // namespace lqmc

@@ -1,0 +1,173 @@
+// Batched GEMM on CDNA4 matrix cores for the slice products and wraps.
+//
+// One launch covers a heterogeneous list of tasks (chains of different N, both
+// spins, several product kinds); each 256-thread workgroup computes one 64x64
+// tile of one task.  Diagonal factors of the DQMC algebra (V = exp(-s*lambda*h),
+// the UDV D splits) are fused: diag(rs) and diag(cs) in the epilogue, diag(ks)
+// on the A tile as it is staged in LDS.  Matrix-core ops: v_mfma_f64_16x16x4f64
+// (FP64) and v_mfma_f32_16x16x4f32 (FP32), 2x2 of them per wavefront per k-step.
+#include "common.h"
+
+namespace lqmc {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+// Accumulator layout of the 16x16 output block: lane holds column lane%16 and
+// four rows, row(lane, r).  FP64: lane/16 + 4r;  FP32: 4*(lane/16) + r.
+template <> struct Mfma<double> {
+  using acc_t = v4d;
+  __device__ static inline acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct Mfma<float> {
+  using acc_t = v4f;
+  __device__ static inline acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static inline int row(int lane, int r) { return ((lane >> 4) << 2) + r; }
+};
+
+__device__ inline double scale_val(int kind, const double* s, int idx) {
+  if (kind == SC_NONE) return 1.0;
+  double v = s[idx];
+  switch (kind) {
+    case SC_PLAIN: return v;
+    case SC_INV: return 1.0 / v;
+    case SC_INV_BIG: return 1.0 / fmax(v, 1.0);
+    case SC_SMALL: return fmin(v, 1.0);
+    default: return 1.0;
+  }
+}
+
+constexpr int BM = 64, BN = 64, BK = 16, PAD = 4;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks, int ntasks) {
+  using acc_t = typename Mfma<T>::acc_t;
+  const int tile = blockIdx.x;
+  int lo = 0, hi = ntasks - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].tile_base <= tile) lo = mid; else hi = mid - 1;
+  }
+  const GemmTask tk = tasks[lo];
+  const int local = tile - tk.tile_base;
+  const int m0 = (local / tk.tiles_n) * BM;
+  const int n0 = (local % tk.tiles_n) * BN;
+  const int M = tk.M, N = tk.N, K = tk.K;
+  const T* __restrict__ A = static_cast<const T*>(tk.A);
+  const T* __restrict__ B = static_cast<const T*>(tk.B);
+  T* __restrict__ C = static_cast<T*>(tk.C);
+
+  __shared__ T As[BK][BM + PAD];
+  __shared__ T Bs[BK][BN + PAD];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  // global->register staging coordinates
+  // A: non-trans: row m = tid/4, k = (tid%4)*4 + q ; trans: k = tid/16, m = (tid%16)*4 + q
+  // B: non-trans: k = tid/16, n = (tid%16)*4 + q ; trans: n = tid/4, k = (tid%4)*4 + q
+  T ra[4], rb[4];
+  auto load_tiles = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int m, k;
+      if (!tk.transA) { m = m0 + (tid >> 2); k = k0 + ((tid & 3) << 2) + q; }
+      else { k = k0 + (tid >> 4); m = m0 + ((tid & 15) << 2) + q; }
+      T v = T(0);
+      if (m < M && k < K) {
+        v = tk.transA ? A[(size_t)k * tk.lda + m] : A[(size_t)m * tk.lda + k];
+        if (tk.ks_kind) v = v * T(scale_val(tk.ks_kind, tk.ks, k));
+      }
+      ra[q] = v;
+      int n, kb;
+      if (!tk.transB) { kb = k0 + (tid >> 4); n = n0 + ((tid & 15) << 2) + q; }
+      else { n = n0 + (tid >> 2); kb = k0 + ((tid & 3) << 2) + q; }
+      T w = T(0);
+      if (n < N && kb < K) w = tk.transB ? B[(size_t)n * tk.ldb + kb] : B[(size_t)kb * tk.ldb + n];
+      rb[q] = w;
+    }
+  };
+  auto store_tiles = [&]() {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (!tk.transA) As[((tid & 3) << 2) + q][tid >> 2] = ra[q];
+      else As[tid >> 4][((tid & 15) << 2) + q] = ra[q];
+      if (!tk.transB) Bs[tid >> 4][((tid & 15) << 2) + q] = rb[q];
+      else Bs[((tid & 3) << 2) + q][tid >> 2] = rb[q];
+    }
+  };
+
+  acc_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+  const int nk = (K + BK - 1) / BK;
+  load_tiles(0);
+  store_tiles();
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    if (kt + 1 < nk) load_tiles((kt + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int kr = kk + (lane >> 4);
+      T a0 = As[kr][wm * 32 + (lane & 15)];
+      T a1 = As[kr][wm * 32 + 16 + (lane & 15)];
+      T b0 = Bs[kr][wn * 32 + (lane & 15)];
+      T b1 = Bs[kr][wn * 32 + 16 + (lane & 15)];
+      acc[0][0] = Mfma<T>::mma(a0, b0, acc[0][0]);
+      acc[0][1] = Mfma<T>::mma(a0, b1, acc[0][1]);
+      acc[1][0] = Mfma<T>::mma(a1, b0, acc[1][0]);
+      acc[1][1] = Mfma<T>::mma(a1, b1, acc[1][1]);
+    }
+    __syncthreads();
+    if (kt + 1 < nk) {
+      store_tiles();
+      __syncthreads();
+    }
+  }
+
+  // epilogue: lane holds rows Mfma<T>::row(lane, r), column lane%16 of each 16x16 block
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+      if (col >= N) continue;
+      const double csv = scale_val(tk.cs_kind, tk.cs, col);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * 32 + i * 16 + Mfma<T>::row(lane, r);
+        if (row >= M) continue;
+        double v = (double)acc[i][j][r];
+        v = tk.alpha * scale_val(tk.rs_kind, tk.rs, row) * v * csv;
+        T* p = C + (size_t)row * tk.ldc + col;
+        if (tk.beta != 0.0) v += tk.beta * (double)(*p);
+        *p = (T)v;
+      }
+    }
+}
+
+void launch_gemm(int fp32, const GemmTask* d_tasks, int ntasks, int ntiles, hipStream_t s) {
+  if (ntasks <= 0 || ntiles <= 0) return;
+  if (fp32)
+    hipLaunchKernelGGL(k_gemm<float>, dim3(ntiles), dim3(256), 0, s, d_tasks, ntasks);
+  else
+    hipLaunchKernelGGL(k_gemm<double>, dim3(ntiles), dim3(256), 0, s, d_tasks, ntasks);
+}
+
+int gemm_tiles(int M, int N, int* tiles_n) {
+  *tiles_n = (N + BN - 1) / BN;
+  return ((M + BM - 1) / BM) * (*tiles_n);
+}
+
+}  // namespace lqmc

@@ -1,0 +1,179 @@
+"""Python handle on the native sweep engine (one context = many chains on one GPU).
+
+``ChainSpec`` carries what the reference's ``LatticeQMC.set_beta`` derives
+(lqmc/lqmc.py:93-117): dtau = beta/L, lambda = arccosh(exp(U dtau/2)) (0 for
+U = 0), E_K = expm(-dtau K); E_K^-1 = expm(+dtau K) is uploaded as well so the
+wraps need no device inversion.
+"""
+import ctypes
+from dataclasses import dataclass, field as dc_field
+
+import numpy as np
+from scipy.linalg import expm
+
+from . import _native as nat
+
+_MODES = {"dqmc": nat.MODE_DQMC, "reference": nat.MODE_REFERENCE}
+_PREC = {"fp64": nat.FP64, "fp32": nat.FP32}
+_RNG = {"replay": nat.RNG_REPLAY, "philox": nat.RNG_PHILOX}
+
+
+def slice_params(kmat, beta, time_steps, u):
+    dtau = beta / time_steps
+    lam = float(np.arccosh(np.exp(u * dtau / 2.0))) if u else 0.0
+    return dtau, lam, expm(-dtau * kmat), expm(dtau * kmat)
+
+
+@dataclass
+class ChainSpec:
+    kmat: np.ndarray            # (N, N) kinetic matrix K = -mu I - t A
+    beta: float
+    time_steps: int
+    u: float
+    field: np.ndarray           # (N, L) int8 +-1, reference layout
+    seed: int = 0
+    expk: np.ndarray = dc_field(default=None, repr=False)
+    expk_inv: np.ndarray = dc_field(default=None, repr=False)
+    lam: float = None
+
+    def __post_init__(self):
+        self.kmat = np.ascontiguousarray(self.kmat, dtype=np.float64)
+        self.field = np.ascontiguousarray(self.field, dtype=np.int8)
+        n = self.kmat.shape[0]
+        if self.kmat.shape != (n, n) or self.field.shape != (n, self.time_steps):
+            raise ValueError("kmat must be (N, N) and field (N, time_steps)")
+        if self.expk is None or self.lam is None:
+            _, lam, ek, eki = slice_params(self.kmat, self.beta, self.time_steps, self.u)
+            self.expk = ek if self.expk is None else self.expk
+            self.expk_inv = eki if self.expk_inv is None else self.expk_inv
+            self.lam = lam if self.lam is None else self.lam
+
+    @property
+    def n_sites(self):
+        return self.kmat.shape[0]
+
+
+class SweepEngine:
+    """Batched DQMC sweeps of independent chains on one GPU.
+
+    mode      "dqmc" (textbook, QR/UDV-stabilised) or "reference" (literal
+              lqmc/lqmc.py:301-347 math, FP64 rank-1, unstabilised)
+    precision "fp64" or "fp32" (storage of G, wraps and flip updates; the
+              stabilisation always runs in FP64)
+    delay     1: rank-1 per accepted flip; k > 1: submatrix blocks of k sites
+    n_stab    slices between FP64 recomputes
+    rng       "replay" (uniforms supplied per sweep) or "philox" (on device)
+    """
+
+    def __init__(self, chains, mode="dqmc", precision="fp64", delay=1, n_stab=10, rng="replay",
+                 trace=False, device=0, obs_capacity=4096, flip_threads=0, graph=False):
+        self.lib = nat.load()
+        self.chains = list(chains)
+        if not self.chains:
+            raise ValueError("at least one chain is required")
+        self.mode, self.precision, self.rng = mode, precision, rng
+        self.delay, self.n_stab, self.trace_on = int(delay), int(n_stab), bool(trace)
+        opt = nat.Options(_MODES[mode], _PREC[precision], int(delay), int(n_stab), _RNG[rng],
+                          int(bool(trace)), int(obs_capacity), int(flip_threads))
+        ns = (ctypes.c_int32 * len(self.chains))(*[c.n_sites for c in self.chains])
+        ls = (ctypes.c_int32 * len(self.chains))(*[c.time_steps for c in self.chains])
+        h = ctypes.c_void_p()
+        nat.check(self.lib.lqmc_create(ctypes.byref(h), int(device), len(self.chains), ns, ls,
+                                       ctypes.byref(opt)), "lqmc_create")
+        self.h = h
+        for i, c in enumerate(self.chains):
+            self._set_chain(i, c)
+        nat.check(self.lib.lqmc_init(self.h), "lqmc_init")
+        if graph:
+            nat.check(self.lib.lqmc_use_graph(self.h, 1), "lqmc_use_graph")
+
+    @staticmethod
+    def _dp(a):
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+    def _set_chain(self, i, c):
+        ek = np.ascontiguousarray(c.expk, dtype=np.float64)
+        eki = np.ascontiguousarray(c.expk_inv, dtype=np.float64)
+        km = np.ascontiguousarray(c.kmat, dtype=np.float64)
+        f = np.ascontiguousarray(c.field, dtype=np.int8)
+        nat.check(self.lib.lqmc_set_chain(self.h, i, self._dp(ek), self._dp(eki), self._dp(km),
+                                          float(c.lam), f.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)),
+                                          int(c.seed) & 0xFFFFFFFFFFFFFFFF), "lqmc_set_chain")
+
+    # -- control ---------------------------------------------------------------------------------
+    def set_uniforms(self, i, u):
+        c = self.chains[i]
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1, c.n_sites * c.time_steps)
+        nat.check(self.lib.lqmc_set_uniforms(self.h, i, self._dp(u), u.shape[0]), "lqmc_set_uniforms")
+
+    def sweep(self, n=1, measure=False, stream=None):
+        nat.check(self.lib.lqmc_sweep(self.h, int(n), int(bool(measure)), stream), "lqmc_sweep")
+
+    def sync(self):
+        nat.check(self.lib.lqmc_sync(self.h), "lqmc_sync")
+
+    def use_graph(self, on=True):
+        nat.check(self.lib.lqmc_use_graph(self.h, int(bool(on))), "lqmc_use_graph")
+
+    # -- results ---------------------------------------------------------------------------------
+    def green(self, i, which=0):
+        n = self.chains[i].n_sites
+        out = np.empty((2, n, n))
+        nat.check(self.lib.lqmc_get_green(self.h, i, self._dp(out), int(which)), "lqmc_get_green")
+        return out
+
+    def field(self, i):
+        c = self.chains[i]
+        out = np.empty((c.n_sites, c.time_steps), dtype=np.int8)
+        nat.check(self.lib.lqmc_get_field(self.h, i, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int8))),
+                  "lqmc_get_field")
+        return out
+
+    def stats(self, i=-1):
+        s = nat.Stats()
+        nat.check(self.lib.lqmc_get_stats(self.h, int(i), ctypes.byref(s)), "lqmc_get_stats")
+        return {k: getattr(s, k) for k, _ in nat.Stats._fields_ if k != "pad"}
+
+    def reset_stats(self):
+        nat.check(self.lib.lqmc_reset_stats(self.h), "lqmc_reset_stats")
+
+    def trace(self, i):
+        c = self.chains[i]
+        nl = c.n_sites * c.time_steps
+        r = np.empty(nl)
+        a = np.empty(nl, dtype=np.uint8)
+        nat.check(self.lib.lqmc_get_trace(self.h, i, self._dp(r), a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))),
+                  "lqmc_get_trace")
+        return r, a.astype(bool)
+
+    def observables(self, i, max_rows=1 << 20):
+        st = self.stats(i)
+        rows_max = int(min(max_rows, max(st["measured"], 1)))
+        out = np.empty((rows_max, 3))
+        rows = ctypes.c_int()
+        nat.check(self.lib.lqmc_get_observables(self.h, i, self._dp(out), rows_max, ctypes.byref(rows)),
+                  "lqmc_get_observables")
+        return out[: rows.value]
+
+    def profile(self, kind):
+        nat.check(self.lib.lqmc_profile(self.h, {"off": -1, "flip": 0, "gemm": 1, "qr": 2, "solve": 3}[kind]),
+                  "lqmc_profile")
+
+    def profile_result(self):
+        avg = ctypes.c_double()
+        n = ctypes.c_int64()
+        w = ctypes.c_double()
+        nat.check(self.lib.lqmc_profile_result(self.h, ctypes.byref(avg), ctypes.byref(n), ctypes.byref(w)),
+                  "lqmc_profile_result")
+        return avg.value, n.value, w.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lqmc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

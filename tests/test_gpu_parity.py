@@ -1,0 +1,144 @@
+"""GPU sweep vs the oracle, through the C ABI (ctypes).
+
+Bars: accept bits identical except ties |ratio - u| < 1e-12; FP64 G within 1e-8
+(absolute, Mode D) or 1e-8 * max(1, max|G|) (Mode R, whose G reaches 1e2..1e3).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from gpu_cases import masked_equal, spec_from, square_case
+from latticeqmc_amd.engine import SweepEngine
+from oracle import dqmc, exact, model, philox
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("name", ["mode_r_2x2", "mode_r_4x4", "mode_r_4x4_u0"])
+def test_mode_r_matches_reference_fixture(golden_dir, name):
+    _say(f"[mode_r] {name}")
+    g = np.load(f"{golden_dir}/{name}.npz")
+    L = int(g["L"])
+    spec = spec_from(g["K"], float(g["beta"]), L, float(g["u"]), g["field0"])
+    eng = SweepEngine([spec], mode="reference", precision="fp64", rng="replay", trace=True)
+    for s in range(g["uniforms"].shape[0]):
+        eng.set_uniforms(0, g["uniforms"][s])
+        eng.sweep(1, measure=True)
+        eng.sync()
+        ratio, acc = eng.trace(0)
+        ok, ties = masked_equal(acc, g["acc"][s], g["ratio"][s], g["uniforms"][s])
+        assert ok, f"sweep {s}: accept bits differ"
+        assert np.abs(ratio - g["ratio"][s]).max() <= 1e-8 * max(1.0, np.abs(g["ratio"][s]).max())
+        assert np.array_equal(eng.field(0), g["field"][s])
+        gl = eng.green(0, which=2)
+        scale = max(1.0, np.abs(g["g_up"][s]).max(), np.abs(g["g_dn"][s]).max())
+        assert np.abs(gl[0] - g["g_up"][s]).max() <= 1e-8 * scale
+        assert np.abs(gl[1] - g["g_dn"][s]).max() <= 1e-8 * scale
+    assert np.abs(eng.green(0, which=1) - g["meas_g"]).max() <= 1e-8 * max(1.0, np.abs(g["meas_g"]).max())
+    eng.close()
+
+
+def _run_dqmc_vs_oracle(side, L, beta, u, n_stab, delay, sweeps, seed=0, precision="fp64"):
+    k, f = square_case(side, L, beta, u, seed)
+    n = side * side
+    o = dqmc.DQMCOracle(k, beta, L, u, f, n_stab=n_stab)
+    eng = SweepEngine([spec_from(k, beta, L, u, f)], mode="dqmc", precision=precision, delay=delay,
+                      n_stab=n_stab, rng="replay", trace=True)
+    g0 = eng.green(0, which=0)
+    init_err = max(np.abs(g0[s] - o.g[s]).max() for s in range(2))
+    rs = np.random.RandomState(100 + seed)
+    worst = 0.0
+    for s in range(sweeps):
+        uni = rs.rand(L * n)
+        res = o.sweep(uni)
+        eng.set_uniforms(0, uni)
+        eng.sweep(1, measure=True)
+        eng.sync()
+        ratio, acc = eng.trace(0)
+        ok, ties = masked_equal(acc, res["accept"], res["ratio"], uni)
+        if precision == "fp64":
+            assert ok, f"sweep {s}: accept bits differ"
+            assert np.array_equal(eng.field(0), o.field)
+        gl = eng.green(0, which=2)
+        gc = eng.green(0, which=0)
+        worst = max(worst, max(np.abs(gl[sp] - res["g_meas"][sp]).max() for sp in range(2)),
+                    max(np.abs(gc[sp] - o.g[sp]).max() for sp in range(2)))
+    st = eng.stats(0)
+    eng.close()
+    return init_err, worst, st
+
+
+@pytest.mark.parametrize("side,L,beta,n_stab,delay", [
+    (2, 16, 2.0, 8, 1), (4, 16, 2.0, 8, 1), (4, 16, 2.0, 8, 4), (4, 16, 2.0, 5, 16),
+    (8, 40, 4.0, 10, 1), (8, 40, 4.0, 10, 8), (6, 20, 2.0, 10, 32),
+])
+def test_dqmc_fp64_matches_oracle(side, L, beta, n_stab, delay):
+    _say(f"[dqmc fp64] side={side} L={L} beta={beta} n_stab={n_stab} delay={delay}")
+    init_err, worst, st = _run_dqmc_vs_oracle(side, L, beta, 4.0, n_stab, delay, sweeps=2)
+    assert init_err < 1e-8
+    assert worst < 1e-8
+    assert st["flags"] == 0
+    assert st["max_err"] < 1e-8
+
+
+def test_dqmc_u0_closed_form():
+    _say("[dqmc u0]")
+    k = model.square_k(4, 4)
+    f = model.random_field(16, 16, 0)
+    eng = SweepEngine([spec_from(k, 2.0, 16, 0.0, f)], mode="dqmc", n_stab=8, rng="philox")
+    eng.sweep(2, measure=True)
+    eng.sync()
+    free = exact.free_g(k, 2.0)
+    g = eng.green(0, which=1)
+    assert np.abs(g[0] - free).max() < 1e-10 and np.abs(g[1] - free).max() < 1e-10
+    st = eng.stats(0)
+    assert st["accepted"] == st["attempted"] == 2 * 16 * 16
+    eng.close()
+
+
+def test_philox_stream_matches_oracle():
+    _say("[philox]")
+    side, L, beta = 4, 16, 2.0
+    k, f = square_case(side, L, beta, 4.0, 1)
+    seed = 987654321
+    o = dqmc.DQMCOracle(k, beta, L, 4.0, f, n_stab=8)
+    spec = spec_from(k, beta, L, 4.0, f, seed=seed)
+    eng = SweepEngine([spec], mode="dqmc", n_stab=8, rng="philox", trace=True)
+    for s in range(2):
+        uni = philox.uniforms(seed, s * L * 16, L * 16)
+        res = o.sweep(uni)
+        eng.sweep(1)
+        eng.sync()
+        ratio, acc = eng.trace(0)
+        ok, _ = masked_equal(acc, res["accept"], res["ratio"], uni)
+        assert ok
+    assert np.array_equal(eng.field(0), o.field)
+    eng.close()
+
+
+def test_batched_uneven_equals_alone():
+    _say("[batched]")
+    cases = [(4, 16, 2.0, 1), (6, 20, 2.0, 2), (2, 8, 1.0, 3), (4, 12, 1.5, 4)]
+    specs = []
+    for side, L, beta, seed in cases:
+        k, f = square_case(side, L, beta, 4.0, seed)
+        specs.append(spec_from(k, beta, L, 3.0 + seed, f, seed=seed))
+    batched = SweepEngine(specs, mode="dqmc", n_stab=5, rng="philox")
+    batched.sweep(3, measure=True)
+    batched.sync()
+    for i, sp in enumerate(specs):
+        alone = SweepEngine([sp], mode="dqmc", n_stab=5, rng="philox")
+        alone.sweep(3, measure=True)
+        alone.sync()
+        assert np.array_equal(alone.green(0, which=1), batched.green(i, which=1)), i
+        assert np.array_equal(alone.green(0, which=0), batched.green(i, which=0)), i
+        assert np.array_equal(alone.field(0), batched.field(i)), i
+        assert alone.stats(0)["accepted"] == batched.stats(i)["accepted"]
+        alone.close()
+    batched.close()

@@ -1,0 +1,58 @@
+"""Host-side pieces: Configuration RNG semantics and the C-ABI library exports (no GPU calls)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from latticeqmc_amd.configuration import Configuration
+from latticeqmc_amd.tools import binned_mean, local_moment
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_configuration_draw_matches_reference_fixture(golden_dir):
+    g = np.load(f"{golden_dir}/mode_r_4x4.npz")
+    np.random.seed(int(g["seed"]))
+    c = Configuration(16, 16)
+    assert c.config.dtype == np.int8
+    assert np.array_equal(c.config, g["field0"])
+    c.update(3, 5)
+    assert c[3, 5] == -g["field0"][3, 5]
+
+
+def test_uniform_stream_vectorised_equals_scalar_draws():
+    a = np.random.RandomState(7)
+    b = np.random.RandomState(7)
+    assert np.array_equal([a.rand() for _ in range(257)], b.rand(257))
+
+
+def test_local_moment_and_binning():
+    g = np.diag([0.25, 0.75])
+    assert np.allclose(local_moment(g, g), [0.75 + 0.75 - 2 * 0.5625, 0.25 + 0.25 - 2 * 0.0625])
+    m, e = binned_mean(np.arange(100.0), 10)
+    assert m == 49.5 and e > 0
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "lqmc_hip.h")).read()
+    return sorted(set(re.findall(r"\b(lqmc_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from latticeqmc_amd import _native
+    if not os.path.exists(_native.LIB_PATH):
+        pytest.fail("liblqmc_hip.so is not built; run python -m latticeqmc_amd.build")
+    lib = ctypes.CDLL(_native.LIB_PATH)
+    decl = _declared_symbols()
+    assert len(decl) >= 18
+    for name in decl:
+        assert hasattr(lib, name), name
+    assert set(_native.EXPORTS) == set(decl)
+
+
+def test_native_load_declares_prototypes():
+    from latticeqmc_amd import _native
+    lib = _native.load()
+    assert lib.lqmc_last_error() is not None
