@@ -1,0 +1,241 @@
+"""Throughput benchmark: spin-flip updates/sec of the DQMC sweep on one MI355X.
+
+Default workload (the configuration the metric is quoted on): 64 independent
+chains of a 16x16 periodic square lattice (N=256), t=1, U=4, mu=0, beta=8,
+dtau=0.1 (L=80), i.i.d. +-1 fields from seeds 0..63, FP64 QR/UDV recompute every
+n_s slices.  One step = one sweep of every chain = 64*256*80 = 1,310,720
+attempted single-spin flips.
+
+Forms timed:
+  chosen  batched chains, submatrix-delayed rank-k updates, FP32 wraps/updates on
+          the matrix cores, FP64 stabilisation, whole sweep replayed from a hipGraph
+  plain   the same chains batched, FP64, rank-1 Sherman-Morrison per accepted
+          flip, no graph
+value = chosen-form updates/sec over exactly --steps sweeps (inputs resident in
+HBM, device synchronised before and after).  The JSON line also carries the
+plain-form rate, their ratio, max|G - G_FP64| over the FP64 recomputes, the
+roofline record of the dominant kernel and the CPU baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"hbm": 8000.0, "fp64": 78.6, "fp32": 157.3}  # GB/s, TFLOP/s (MI355X dense)
+
+CONFIGS = {
+    "c4": dict(desc="64 chains 16x16 U=4 beta=8 dtau=0.1 L=80", sides=[16] * 64, u=[4.0] * 64,
+               beta=[8.0] * 64, L=[80] * 64, seeds=list(range(64))),
+    "c2": dict(desc="1 chain 8x8 U=4 beta=4 dtau=0.1 L=40", sides=[8], u=[4.0], beta=[4.0], L=[40], seeds=[0]),
+    "c5": dict(desc="32 chains 24x24 U=6 beta=10 dtau=0.1 L=100", sides=[24] * 32, u=[6.0] * 32,
+               beta=[10.0] * 32, L=[100] * 32, seeds=list(range(32))),
+}
+
+
+def c3_config():
+    rs = np.random.RandomState(0)
+    sides = rs.choice([6, 8, 10], size=256).tolist()
+    u = rs.uniform(2.0, 8.0, size=256).tolist()
+    beta = rs.choice([2.0, 4.0, 6.0], size=256).tolist()
+    L = [int(round(b / 0.1)) for b in beta]
+    return dict(desc="256 chains side in {6,8,10}, U~U[2,8], beta in {2,4,6}, dtau=0.1", sides=sides, u=u,
+                beta=beta, L=L, seeds=list(range(256)))
+
+
+CONFIGS["c3"] = None
+
+
+def build_specs(cfg):
+    from latticeqmc_amd.engine import ChainSpec
+    from latticeqmc_amd.hubbard import HubbardModel
+    specs = []
+    kcache = {}
+    for side, u, beta, L, seed in zip(cfg["sides"], cfg["u"], cfg["beta"], cfg["L"], cfg["seeds"]):
+        if side not in kcache:
+            m = HubbardModel(u=u, t=1.0, mu=0.0)
+            m.build_square(side)
+            kcache[side] = m.ham_kinetic()
+        k = kcache[side]
+        rs = np.random.RandomState(seed)
+        field = (2 * rs.randint(0, 2, size=(side * side, L)) - 1).astype(np.int8)
+        specs.append(ChainSpec(k, beta, L, u, field, seed=seed))
+    return specs
+
+
+def sync_all():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def time_form(specs, steps, warmup, profile_kind=None, **kw):
+    from latticeqmc_amd.engine import SweepEngine
+    eng = SweepEngine(specs, rng="philox", **kw)
+    upd = sum(s.n_sites * s.time_steps for s in specs)
+    eng.sweep(warmup, measure=False)
+    eng.sync()
+    eng.reset_stats()
+    sync_all()
+    t0 = time.perf_counter()
+    eng.sweep(steps, measure=True)
+    eng.sync()
+    sync_all()
+    dt = time.perf_counter() - t0
+    st = eng.stats()
+    prof = None
+    if profile_kind:
+        eng.use_graph(False)
+        eng.profile(profile_kind)
+        eng.sweep(max(1, min(steps, 2)), measure=True)
+        eng.sync()
+        prof = eng.profile_result()
+        eng.profile("off")
+    eng.close()
+    return dict(ms_per_step=1e3 * dt / steps, ups=upd * steps / dt, stats=st, prof=prof, upd_per_step=upd)
+
+
+def cpu_baseline(cfg, budget_s=15.0, procs=None):
+    """Reference-speed CPU sweep (faithful restatement, the reference's j,k loop and
+    per-attempt mean/var) on a bounded sample, one chain per process like the
+    reference's chain pool (lqmc/multiprocessing.py:252-289)."""
+    import multiprocessing as mp
+    procs = procs or min(16, os.cpu_count() or 1)
+    n = cfg["sides"][0] ** 2
+    # one slice of a 16x16 chain is ~256 attempts at ~20 upd/s per core in the faithful form
+    slices = 1
+    ctx = mp.get_context("fork")
+    with ctx.Pool(procs, initializer=_pool_init) as pool:
+        t0 = time.perf_counter()
+        counts = pool.map(_cpu_worker, [(cfg["sides"][c % len(cfg["sides"])], cfg["u"][0], cfg["beta"][0],
+                                         cfg["L"][0], c, slices) for c in range(procs)])
+        dt = time.perf_counter() - t0
+    total = sum(counts)
+    return dict(value=total / dt, unit="spin-flip updates/s", cores=procs, kind="port",
+                sample=f"faithful restatement of lqmc.py:301-347 (Python j,k loop, per-attempt mean/var), "
+                       f"{slices} slice x {n} sites per process incl. the sweep-start inversion, "
+                       f"{procs} processes, {total} attempts in {dt:.1f} s")
+
+
+def _pool_init():
+    os.environ["OPENBLAS_NUM_THREADS"] = "1"
+    try:
+        from threadpoolctl import threadpool_limits
+        threadpool_limits(1)
+    except Exception:
+        pass
+
+
+def _cpu_worker(args):
+    side, u, beta, L, seed, slices = args
+    from oracle import mode_r, model
+    k = model.square_k(side, side)
+    _, lam, ek, _ = model.params(k, beta, L, u)
+    f = model.random_field(side * side, L, seed)
+    uni = np.random.RandomState(seed + 1000).rand(side * side * L)
+    return mode_r.sweep_faithful(f, ek, lam, uni, max_slices=slices)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c4")
+    ap.add_argument("--delay", type=int, default=32)
+    ap.add_argument("--n-stab", type=int, default=10)
+    ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-kind", default="flip", help="kernel kind timed with HIP events: flip|gemm|qr|solve")
+    ap.add_argument("--form", default="both", help="chosen|plain|both")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: every box of this build has one GPU and the chains of the "
+              "workload shard trivially across GPUs (see the sharding note); run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    cfg = c3_config() if args.config == "c3" else CONFIGS[args.config]
+    specs = build_specs(cfg)
+    res = {}
+    if args.form in ("chosen", "both"):
+        res["chosen"] = time_form(specs, args.steps, args.warmup, profile_kind=args.profile_kind,
+                                  mode="dqmc", precision=args.precision, delay=args.delay,
+                                  n_stab=args.n_stab, graph=not args.no_graph)
+    if args.form in ("plain", "both") and not args.no_plain:
+        res["plain"] = time_form(specs, args.steps, args.warmup, profile_kind=None, mode="dqmc",
+                                 precision="fp64", delay=1, n_stab=args.n_stab, graph=False)
+    main_form = res.get("chosen") or res.get("plain")
+    out = {
+        "metric": "spin-flip updates/sec, chosen form vs FP64 plain on 1 GPU, max|dG|<=1e-6",
+        "value": main_form["ups"],
+        "unit": "spin-flip updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_form["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision if "chosen" in res else "fp64",
+        "data": "synthetic (i.i.d. +-1 HS fields from seeds, Philox uniforms)",
+        "config": {"workload": cfg["desc"], "chains": len(specs),
+                   "updates_per_step": main_form["upd_per_step"], "delay": args.delay,
+                   "n_stab": args.n_stab, "graph": not args.no_graph},
+    }
+    if "chosen" in res:
+        st = res["chosen"]["stats"]
+        out["max_dG_vs_fp64"] = st["max_err"]
+        out["acceptance"] = st["accepted"] / max(st["attempted"], 1)
+    if "plain" in res:
+        out["plain_fp64_value"] = res["plain"]["ups"]
+        out["plain_max_dG"] = res["plain"]["stats"]["max_err"]
+        if "chosen" in res:
+            out["speedup_vs_plain"] = res["chosen"]["ups"] / res["plain"]["ups"]
+    prof = main_form.get("prof")
+    if prof:
+        avg_ms, launches, work = prof
+        out["roofline"] = roofline(args.profile_kind, avg_ms, launches, work, specs, args, main_form)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg)
+    print(json.dumps(out))
+
+
+def roofline(kind, avg_ms, launches, work, specs, args, form):
+    """Algorithmic traffic/flops of the profiled kernel kind per launch over its mean duration."""
+    st = form["stats"]
+    acc = st["accepted"] / max(st["attempted"], 1)
+    esz = 4 if args.precision == "fp32" else 8
+    if kind == "flip":
+        # per attempted flip: G_ii of both spins (2*esz); per accepted flip in the delayed
+        # form, the rank-k product streams G once per block: 2 spins * N^2 * esz * (read+write) / k
+        per_launch_flips = work / max(launches, 1)
+        n = specs[0].n_sites
+        k = max(args.delay, 1)
+        bytes_per_flip = 2 * esz + acc * 2 * 2 * n * n * esz / k * min(1.0, 1.0)
+        bytes_launch = per_launch_flips * bytes_per_flip
+        achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+        return {"bound": "hbm", "kernel": "flip", "achieved": achieved, "peak": PEAK["hbm"], "unit": "GB/s",
+                "frac": achieved / PEAK["hbm"], "traffic": None, "avg_ms": avg_ms, "launches": launches,
+                "bytes_per_launch": bytes_launch}
+    flops_launch = work / max(launches, 1)
+    achieved = flops_launch / (avg_ms * 1e-3) / 1e12
+    peak = PEAK["fp32"] if (kind == "gemm" and args.precision == "fp32") else PEAK["fp64"]
+    return {"bound": "mfma", "kernel": kind, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": None, "avg_ms": avg_ms, "launches": launches,
+            "flops_per_launch": flops_launch}
+
+
+if __name__ == "__main__":
+    main()

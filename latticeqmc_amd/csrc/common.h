@@ -1,6 +1,10 @@
 // Device-side descriptors shared by the engine (host) and the kernels.
 #pragma once
+#ifndef LQMC_CPU_EMU
 #include <hip/hip_runtime.h>
+// Dynamic LDS of a kernel (redirected to a host buffer by the CPU bounds-check build).
+#define LQMC_DYNAMIC_LDS(type, name) extern __shared__ type name[]
+#endif
 #include <stdint.h>
 
 namespace lqmc {
@@ -25,6 +29,7 @@ struct ChainDev {
   double* tr_ratio;    // [L][N] trace
   uint8_t* tr_acc;     // [L][N] trace
   double* obs;         // [obs_cap][3]
+  void* dws;           // delayed-update operands: [2 spins][P: Np x pld | Z: K4 x Np], form precision
   int obs_cap;
   int pad1;
 };
@@ -64,12 +69,23 @@ struct GemmTask {
   double alpha, beta;
 };
 
-// Pivoted Householder QR: W = Q diag(D) T (W destroyed).
+// Pre-pivoted blocked Householder QR: W = Q diag(D) V'' with V'' written over W.
 struct QrTask {
-  double* W;
-  double* Q;
-  double* D;
-  double* T;
+  double* W;   // in: matrix; out: V'' = D^-1 R P^T
+  double* Q;   // out: orthogonal factor
+  double* D;   // out: |diag R|
+  double* A;   // scratch N x N (holds R and the reflectors)
+  int* perm;   // optional out: pivot order
+  int N;
+  int chain;
+};
+
+// Solve V'' Y = Z for the V'' and pivot order of a QrTask (Z destroyed).
+struct TrsmTask {
+  const double* V;
+  const int* perm;
+  double* Z;
+  double* Y;
   int N;
   int chain;
 };

@@ -29,7 +29,8 @@
 namespace lqmc {
 void launch_gemm(int fp32, const GemmTask* d_tasks, int ntasks, int ntiles, hipStream_t s);
 int gemm_tiles(int M, int N, int* tiles_n);
-void launch_qr(const QrTask* d, int n, int nmax, ChainState* st, hipStream_t s);
+void launch_qr_blocked(const QrTask* d, int n, int nmax, ChainState* st, hipStream_t s);
+void launch_trsm(const TrsmTask* d, int n, int nmax, ChainState* st, hipStream_t s);
 void launch_solve(const SolveTask* d, int n, int nmax, ChainState* st, hipStream_t s);
 void launch_finish(int fp32, const FinishTask* d, int n, int nmax, ChainState* st, hipStream_t s);
 void launch_identity(double* const* ptrs, const int* ns, int n, hipStream_t s);
@@ -37,7 +38,7 @@ void launch_expv(const ChainDev* chains, int n, hipStream_t s);
 void launch_flip(int fp32, int delayed, const FlipArgs& a, int nchains, int threads, size_t shmem,
                  hipStream_t s);
 size_t flip_delayed_shmem(int nmax, int k, int threads, int fp32);
-int flip_prepare(size_t shmem);
+size_t flip_delayed_workspace(int n, int k, int fp32);
 }  // namespace lqmc
 
 using namespace lqmc;
@@ -58,7 +59,7 @@ static int fail(int code, const std::string& msg) {
 
 namespace {
 
-enum OpKind { OP_GEMM = 0, OP_QR, OP_SOLVE, OP_FINISH, OP_FLIP, OP_EXPV, OP_IDENT };
+enum OpKind { OP_GEMM = 0, OP_QR, OP_SOLVE, OP_FINISH, OP_FLIP, OP_EXPV, OP_IDENT, OP_TRSM };
 enum ProfKind { PK_FLIP = 0, PK_GEMM = 1, PK_QR = 2, PK_SOLVE = 3 };
 
 struct Op {
@@ -90,8 +91,11 @@ struct Chain {
   double *LU[2] = {}, *LD[2] = {}, *LV[2] = {};
   double *RU[2] = {}, *RD[2] = {}, *RV[2][2] = {};
   double *Tq[2] = {}, *aug[2] = {};
+  double *Dx[2] = {}, *Zb[2] = {};
+  int* px[2] = {};
   double *eye = nullptr, *ones = nullptr;
   void* Tw[2] = {};
+  void* dws = nullptr;    // delayed-update operands
   double* uni = nullptr;  // separate allocation (replay)
   int64_t uni_cap = 0;
 };
@@ -172,6 +176,11 @@ static void layout_chain(const lqmc_ctx* x, Chain& c, char* base, size_t& off) {
   }
   carve(base, off, c.eye, n2);
   carve(base, off, c.ones, N);
+  if (x->opt.delay > 1) {
+    char* w = nullptr;
+    carve(base, off, w, flip_delayed_workspace(c.N, x->opt.delay, fp32 ? 1 : 0));
+    c.dws = w;
+  }
   for (int s = 0; s < 2; ++s) {
     carve(base, off, c.Wa[s], n2);
     carve(base, off, c.Wb[s], n2);
@@ -188,6 +197,9 @@ static void layout_chain(const lqmc_ctx* x, Chain& c, char* base, size_t& off) {
       carve(base, off, c.RV[s][0], n2);
       carve(base, off, c.RV[s][1], n2);
       carve(base, off, c.Tq[s], n2);
+      carve(base, off, c.Dx[s], N);
+      carve(base, off, c.Zb[s], n2);
+      carve(base, off, c.px[s], N);
     }
   }
 }
@@ -241,6 +253,14 @@ struct Builder {
     ops.push_back(Op{OP_SOLVE, 0, push(tasks), (int)tasks.size(), 0, nm, 0, fl});
     tasks.clear();
   }
+  void trsm(std::vector<Op>& ops, std::vector<TrsmTask>& tasks) {
+    if (tasks.empty()) return;
+    int nm = 0;
+    double fl = 0;
+    for (auto& t : tasks) { nm = std::max(nm, t.N); fl += (double)t.N * t.N * t.N; }
+    ops.push_back(Op{OP_TRSM, 0, push(tasks), (int)tasks.size(), 0, nm, 0, fl});
+    tasks.clear();
+  }
   void finish(std::vector<Op>& ops, std::vector<FinishTask>& tasks, int fp32) {
     if (tasks.empty()) return;
     int nm = 0;
@@ -274,6 +294,7 @@ static void build_program(lqmc_ctx* x) {
   std::vector<GemmTask> g1, g2;
   std::vector<QrTask> q;
   std::vector<SolveTask> sv;
+  std::vector<TrsmTask> ts;
   std::vector<FinishTask> fin;
 
   // ---- init: scale tables and identity matrices ----
@@ -378,16 +399,19 @@ static void build_program(lqmc_ctx* x) {
           const bool bottom = (m == c.nb - 1);
           for (int s = 0; s < 2; ++s) {
             const size_t n2 = (size_t)c.N * c.N;
-            q.push_back(QrTask{wcur[ci * 2 + s], c.LU[s] + m * n2, c.LD[s] + (size_t)m * c.N, c.Tq[s], c.N, ci});
+            q.push_back(QrTask{wcur[ci * 2 + s], c.LU[s] + m * n2, c.LD[s] + (size_t)m * c.N, c.Tq[s], nullptr,
+                               c.N, ci});
             const double* vin = bottom ? c.eye : c.LV[s] + (m + 1) * n2;
-            g2.push_back(mk(c.Tq[s], c.N, 0, vin, c.N, 0, c.LV[s] + m * n2, c.N, c.N));
+            g2.push_back(mk(wcur[ci * 2 + s], c.N, 0, vin, c.N, 0, c.LV[s] + m * n2, c.N, c.N));
           }
         }
         b.qr(ops, q);
         b.gemm(ops, g2, 0);
       }
     };
-    // stable G from left entry (or identity) and right factors; writes Gst and finishes into G
+    // stable G from the left entry (or identity) and the right factors:
+    //   X = Db_L^-1 (U_L^T U_R) Db_R^-1 + Ds_L (V_L V_R^T) Ds_R ;  X = Qx Dx Vx   (QR)
+    //   Z = Dx^-1 Qx^T Db_L^-1 U_L^T ;  Vx Y = Z ;  G = U_R Db_R^-1 Y
     auto recompute = [&](std::vector<Op>& ops, std::vector<int>& which, std::vector<int>& mleft,
                          std::vector<int>& rvsel, bool right_identity, int compare) {
       for (size_t k = 0; k < which.size(); ++k) {
@@ -401,25 +425,41 @@ static void build_program(lqmc_ctx* x) {
           const double* UR = right_identity ? c.eye : c.RU[s];
           const double* DR = right_identity ? c.ones : c.RD[s];
           const double* VR = right_identity ? c.eye : c.RV[s][rvsel[k]];
-          GemmTask a = mk(UL, c.N, 1, UR, c.N, 0, c.aug[s], 2 * c.N, c.N);
+          GemmTask a = mk(UL, c.N, 1, UR, c.N, 0, c.aug[s], c.N, c.N);
           a.rs = DL; a.rs_kind = SC_INV_BIG; a.cs = DR; a.cs_kind = SC_INV_BIG;
           g1.push_back(a);
-          GemmTask bb = mk(VL, c.N, 0, VR, c.N, 1, c.aug[s], 2 * c.N, c.N);
+          GemmTask bb = mk(VL, c.N, 0, VR, c.N, 1, c.aug[s], c.N, c.N);
           bb.rs = DL; bb.rs_kind = SC_SMALL; bb.cs = DR; bb.cs_kind = SC_SMALL; bb.beta = 1.0;
           g2.push_back(bb);
-          sv.push_back(SolveTask{c.aug[s], UL, DL, nullptr, c.N, 0, which[k], 0});
+          q.push_back(QrTask{c.aug[s], c.aug[s] + n2, c.Dx[s], c.Tq[s], c.px[s], c.N, which[k]});
         }
       }
       b.gemm(ops, g1, 0);
       b.gemm(ops, g2, 0);
-      b.solve(ops, sv);
+      b.qr(ops, q);
+      for (size_t k = 0; k < which.size(); ++k) {
+        Chain& c = x->ch[which[k]];
+        const size_t n2 = (size_t)c.N * c.N;
+        for (int s = 0; s < 2; ++s) {
+          const bool lid = mleft[k] < 0;
+          const double* UL = lid ? c.eye : c.LU[s] + mleft[k] * n2;
+          const double* DL = lid ? c.ones : c.LD[s] + (size_t)mleft[k] * c.N;
+          GemmTask z = mk(c.aug[s] + n2, c.N, 1, UL, c.N, 1, c.Zb[s], c.N, c.N);
+          z.rs = c.Dx[s]; z.rs_kind = SC_INV;
+          z.ks = DL; z.ks_kind = SC_INV_BIG;
+          g1.push_back(z);
+          ts.push_back(TrsmTask{c.aug[s], c.px[s], c.Zb[s], c.aug[s] + n2, c.N, which[k]});
+        }
+      }
+      b.gemm(ops, g1, 0);
+      b.trsm(ops, ts);
       for (size_t k = 0; k < which.size(); ++k) {
         Chain& c = x->ch[which[k]];
         const size_t n2 = (size_t)c.N * c.N;
         for (int s = 0; s < 2; ++s) {
           const double* UR = right_identity ? c.eye : c.RU[s];
           const double* DR = right_identity ? c.ones : c.RD[s];
-          GemmTask g = mk(UR, c.N, 0, c.aug[s] + c.N, 2 * c.N, 0, c.Gst + s * n2, c.N, c.N);
+          GemmTask g = mk(UR, c.N, 0, c.aug[s] + n2, c.N, 0, c.Gst + s * n2, c.N, c.N);
           g.ks = DR; g.ks_kind = SC_INV_BIG;
           g1.push_back(g);
           fin.push_back(FinishTask{Gp(c, s), c.Gst + s * n2, c.N, which[k], compare, 0});
@@ -484,9 +524,9 @@ static void build_program(lqmc_ctx* x) {
         Chain& c = x->ch[ci];
         const int nxt = rvcur[ci] < 0 ? 0 : 1 - rvcur[ci];
         for (int s = 0; s < 2; ++s) {
-          q.push_back(QrTask{wcur[ci * 2 + s], c.RU[s], c.RD[s], c.Tq[s], c.N, ci});
+          q.push_back(QrTask{wcur[ci * 2 + s], c.RU[s], c.RD[s], c.Tq[s], nullptr, c.N, ci});
           const double* vin = rvcur[ci] < 0 ? c.eye : c.RV[s][rvcur[ci]];
-          g1.push_back(mk(c.Tq[s], c.N, 0, vin, c.N, 0, c.RV[s][nxt], c.N, c.N));
+          g1.push_back(mk(wcur[ci * 2 + s], c.N, 0, vin, c.N, 0, c.RV[s][nxt], c.N, c.N));
         }
         rvcur[ci] = nxt;
         ml.push_back(t == c.L - 1 ? -1 : (t + 1) / ns);
@@ -511,7 +551,7 @@ static void run_ops(lqmc_ctx* x, const std::vector<Op>& ops, int measure, hipStr
     if (op.kind == OP_FLIP) pk = PK_FLIP;
     else if (op.kind == OP_GEMM) pk = PK_GEMM;
     else if (op.kind == OP_QR) pk = PK_QR;
-    else if (op.kind == OP_SOLVE) pk = PK_SOLVE;
+    else if (op.kind == OP_SOLVE || op.kind == OP_TRSM) pk = PK_SOLVE;
     const bool rec = profile && pk == x->prof.kind;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (rec) {
@@ -522,7 +562,8 @@ static void run_ops(lqmc_ctx* x, const std::vector<Op>& ops, int measure, hipStr
     const char* d = x->d_desc + op.off;
     switch (op.kind) {
       case OP_GEMM: launch_gemm(op.fp32, reinterpret_cast<const GemmTask*>(d), op.count, op.ntiles, s); break;
-      case OP_QR: launch_qr(reinterpret_cast<const QrTask*>(d), op.count, op.nmax, x->d_state, s); break;
+      case OP_QR: launch_qr_blocked(reinterpret_cast<const QrTask*>(d), op.count, op.nmax, x->d_state, s); break;
+      case OP_TRSM: launch_trsm(reinterpret_cast<const TrsmTask*>(d), op.count, op.nmax, x->d_state, s); break;
       case OP_SOLVE: launch_solve(reinterpret_cast<const SolveTask*>(d), op.count, op.nmax, x->d_state, s); break;
       case OP_FINISH: launch_finish(op.fp32, reinterpret_cast<const FinishTask*>(d), op.count, op.nmax, x->d_state, s); break;
       case OP_EXPV: launch_expv(x->d_chains, x->nchains, s); break;
@@ -556,6 +597,7 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
   if (o.precision != LQMC_FP64 && o.precision != LQMC_FP32) return fail(LQMC_EINVAL, "unknown precision");
   if (o.rng != LQMC_RNG_REPLAY && o.rng != LQMC_RNG_PHILOX) return fail(LQMC_EINVAL, "unknown rng");
   if (o.delay < 1 || o.delay > 64) return fail(LQMC_EINVAL, "delay must be in 1..64");
+  if (o.delay > 1 && o.delay % 4 != 0) return fail(LQMC_EINVAL, "delay must be 1 or a multiple of 4");
   if (o.n_stab < 1) return fail(LQMC_EINVAL, "n_stab must be >= 1");
   if (o.mode == LQMC_MODE_REFERENCE && (o.precision != LQMC_FP64 || o.delay != 1))
     return fail(LQMC_EINVAL, "reference mode runs the FP64 rank-1 sweep only");
@@ -580,8 +622,10 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
     x->nmax = std::max(x->nmax, (int)n_sites[c]);
     x->lmax = std::max(x->lmax, (int)time_steps[c]);
   }
-  int maxsh = 0;
-  if (hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) maxsh = 65536;
+  // The flip kernels keep their dynamic LDS within 64 KB per workgroup (the
+  // delayed form's rank-k operands live in a per-chain global workspace), which
+  // bounds the supported delay for a given thread count.
+  const int maxsh = 65536;
   const bool fp32 = o.precision == LQMC_FP32;
   if (o.flip_threads > 0) x->flip_threads = o.flip_threads;
   else x->flip_threads = x->nmax >= 128 ? 1024 : 256;
@@ -636,6 +680,7 @@ static void refresh_chain_desc(lqmc_ctx* x, int ci) {
   d.field = c.field; d.ev = c.ev; d.evi = c.evi; d.G = c.G;
   d.Gacc = c.Gacc; d.Glast = c.Glast; d.hop = c.hop; d.uni = c.uni;
   d.tr_ratio = c.tr_ratio; d.tr_acc = c.tr_acc; d.obs = c.obs; d.obs_cap = x->opt.obs_capacity;
+  d.dws = c.dws;
   x->h_chains[ci] = d;
 }
 
@@ -740,7 +785,6 @@ int lqmc_init(lqmc_ctx* x) {
   int rc = upload_program(x);
   if (rc) return rc;
   HIPCHK(hipMemcpy(x->d_chains, x->h_chains.data(), sizeof(ChainDev) * x->nchains, hipMemcpyHostToDevice));
-  if (x->flip_shmem > 65536) (void)flip_prepare(x->flip_shmem);
   run_ops(x, x->init_ops, 0, x->stream, false);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(x->stream));

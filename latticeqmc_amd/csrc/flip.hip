@@ -132,7 +132,7 @@ __global__ __launch_bounds__(1024) void k_flip_rank1(FlipArgs a) {
   const size_t n2 = (size_t)N * N;
   T* __restrict__ G = static_cast<T*>(ch.G);
   int8_t* fld = ch.field + (size_t)l * N;
-  extern __shared__ double sm[];
+  LQMC_DYNAMIC_LDS(double, sm);
   double* xv = sm;          // [2][N] column (Mode D) / e vector (Mode R)
   double* yv = sm + 2 * N;  // [2][N] row (Mode D) / c vector (Mode R)
   double* red = sm + 4 * N; // blockDim
@@ -195,14 +195,20 @@ __global__ __launch_bounds__(1024) void k_flip_rank1(FlipArgs a) {
       }
     }
     __syncthreads();
-    for (size_t e = threadIdx.x; e < 2 * n2; e += blockDim.x) {
-      const int sp = (int)(e / n2);
-      const size_t r = e % n2;
-      const int j = (int)(r / N), k = (int)(r % N);
-      double p;
-      if (ref) p = xv[sp * N + j] * yv[sp * N + k];
-      else p = coef[sp] * (xv[sp * N + j] * yv[sp * N + k]);
-      G[e] = (T)((double)G[e] - p);
+    {
+      const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+      for (int row = threadIdx.x >> 6; row < 2 * N; row += nw) {
+        const int sp = row >= N ? 1 : 0;
+        const int j = row - sp * N;
+        const double xj = xv[sp * N + j];
+        const double cj = coef[sp];
+        T* Gr = G + sp * n2 + (size_t)j * N;
+        const double* y = yv + sp * N;
+        for (int k = lane; k < N; k += 64) {
+          const double p = ref ? xj * y[k] : cj * (xj * y[k]);
+          Gr[k] = (T)((double)Gr[k] - p);
+        }
+      }
     }
     __syncthreads();
     if (threadIdx.x == 0) fld[i] = (int8_t)(-s);
@@ -260,6 +266,75 @@ __device__ void rank_update(T* G, int N, int Np, const T* P, int pld, const T* Z
   }
 }
 
+// Orders LDS traffic between the lanes of one wavefront (the serial phase runs in wave 0).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Per block of k sites (k a multiple of 4):
+//  1. the k x k diagonal blocks of both spins, the block's field entries and its
+//     uniforms go to LDS;
+//  2. wave 0 runs the k attempts on the blocks (rank-1 Sherman-Morrison on k x k),
+//     recording for accepted j the block parts of x_j = c_j G^(j)[:, a_j] and
+//     y_j = e_{a_j} - G^(j)[a_j, :]^T;
+//  3. with U[p][j] = y_p(a_j), L[p][j] = x_p(a_j) (p < j), the full factors are
+//     X = G0[:, A] C (I + U C)^-1 and Y = (E_A - G0[A, :]^T) (I - L)^-1; wave 0
+//     forms the unit-triangular inverses and scatters Mx = C (I + U C)^-1 and
+//     My = (I - L)^-1 onto the block's rows (k x m, zero on rejected sites);
+//  4. P = -G0[:, B] Mx and Z = My^T (E_B - G0[B, :]) on the matrix cores, from
+//     contiguous strips of G0;
+//  5. G <- G + P Z on the matrix cores (= G0 - sum_j x_j y_j^T).
+template <typename T>
+__device__ void strip_products(const T* G, int N, int Np, int b0, int kb, int m, int K4, const double* Mx,
+                               const double* My, int ldm, T* P, int pld, T* Z) {
+  using acc_t = typename MfmaOp<T>::acc_t;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int jt = (m + 15) >> 4;         // 16-wide tiles over the accepted columns
+  const int rt = Np >> 4;
+  // P tiles: (rt x jt);  Z tiles: (jt x rt)
+  for (int tile = wave; tile < 2 * rt * jt; tile += nw) {
+    const bool isP = tile < rt * jt;
+    const int tt = isP ? tile : tile - rt * jt;
+    acc_t acc = acc_t{0, 0, 0, 0};
+    if (isP) {
+      const int r0 = (tt / jt) << 4, j0 = (tt % jt) << 4;
+      const int r = r0 + (lane & 15), j = j0 + (lane & 15);
+      for (int kk = 0; kk < kb; kk += 4) {
+        const int q = kk + (lane >> 4);
+        const T x = (r < N && q < kb) ? G[(size_t)r * N + b0 + q] : T(0);
+        const T y = (q < kb && j < m) ? (T)Mx[q * ldm + j] : T(0);
+        acc = MfmaOp<T>::mma(x, y, acc);
+      }
+      const int col = j0 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = r0 + MfmaOp<T>::row(lane, v);
+        if (col < K4) P[(size_t)row * pld + col] = -acc[v];
+      }
+    } else {
+      const int j0 = (tt / rt) << 4, c0 = (tt % rt) << 4;
+      const int j = j0 + (lane & 15), cc = c0 + (lane & 15);
+      for (int kk = 0; kk < kb; kk += 4) {
+        const int q = kk + (lane >> 4);
+        const T x = (q < kb && j < m) ? (T)My[q * ldm + j] : T(0);
+        T y = T(0);
+        if (q < kb && cc < N) {
+          y = -G[(size_t)(b0 + q) * N + cc];
+          if (cc == b0 + q) y = y + T(1);
+        }
+        acc = MfmaOp<T>::mma(x, y, acc);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = j0 + MfmaOp<T>::row(lane, v);
+        if (row < K4) Z[(size_t)row * Np + cc] = acc[v];
+      }
+    }
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(1024) void k_flip_delayed(FlipArgs a) {
   const int c = blockIdx.x;
@@ -269,30 +344,32 @@ __global__ __launch_bounds__(1024) void k_flip_delayed(FlipArgs a) {
   const int l = ch.L - 1 - a.t;
   const int K = a.delay;
   const int Np = (N + 15) & ~15;
-  const int pld = ((K + 3) & ~3) + 1;  // P row stride >= padded rank, +1 against bank conflicts
+  const int K16 = (K + 15) & ~15;
+  const int pld = K16 + 1;
   const size_t n2 = (size_t)N * N;
   T* __restrict__ G = static_cast<T*>(ch.G);
   int8_t* fld = ch.field + (size_t)l * N;
+  // workspace per spin: P (Np x pld) then Z (K16 x Np), form precision
+  const size_t spT = (size_t)Np * pld + (size_t)K16 * Np;
+  T* wsT = static_cast<T*>(ch.dws);
 
-  extern __shared__ double sm[];
-  double* red = sm;                        // blockDim
-  double* alpha = red + blockDim.x;        // [2][K]
-  double* gam = alpha + 2 * K;             // [K][2K]
-  int* aidx = reinterpret_cast<int*>(gam + 2 * K * K);  // [K]
-  double* uni = reinterpret_cast<double*>(aidx + ((K + 1) & ~1));  // union region
-  // serial phase view
-  double* small = uni;                     // [2][K][K]
-  double* xs = small + 2 * K * K;          // [2][K]
+  LQMC_DYNAMIC_LDS(double, sm);
+  double* red = sm;                        // blockDim (epilogue reductions)
+  double* small = red + blockDim.x;        // [2][K][K]; later the triangular-inverse scratch
+  double* Xsb = small + 2 * K * K;         // [2][K][K]: x_j block parts, later Mx scattered
+  double* Ysb = Xsb + 2 * K * K;           // [2][K][K]: y_j block parts, later My scattered
+  double* xs = Ysb + 2 * K * K;            // [2][K]
   double* ys = xs + 2 * K;                 // [2][K]
-  // update phase view
-  T* P = reinterpret_cast<T*>(uni);        // [Np][pld]
-  T* Z = P + (size_t)Np * pld;             // [K4][Np]
-  __shared__ double coef[2];
-  __shared__ int piv_s;
+  double* cf = ys + 2 * K;                 // [2][K] coefficients of the accepts
+  double* ub = cf + 2 * K;                 // [K] uniforms of the block
+  int* aidx = reinterpret_cast<int*>(ub + K);  // [K] block-local site of each accept
+  int* sb = aidx + K;                          // [K] field entries of the block
+  __shared__ int m_sh;
 
   const unsigned long long sweep = a.st[c].sweep;
   unsigned long long nacc = 0;
   bool underrun = false, bad = false;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 
   for (int b0 = 0; b0 < N; b0 += K) {
     const int kb = min(K, N - b0);
@@ -301,154 +378,129 @@ __global__ __launch_bounds__(1024) void k_flip_delayed(FlipArgs a) {
       const int p = r / kb, q = r % kb;
       small[sp * K * K + p * K + q] = (double)G[sp * n2 + (size_t)(b0 + p) * N + b0 + q];
     }
-    __syncthreads();
-    int m = 0;
-    for (int jj = 0; jj < kb; ++jj) {
-      const int i = b0 + jj;
-      const int s = fld[i];
-      const double gu = small[jj * K + jj];
-      const double gd = small[K * K + jj * K + jj];
-      const double ep = s > 0 ? ch.e2p : ch.e2m;
-      const double em = s > 0 ? ch.e2m : ch.e2p;
-      const double d_up = 1.0 + (1.0 - gu) * (ep - 1.0);
-      const double d_dn = 1.0 + (1.0 - gd) * (em - 1.0);
-      const double ratio = d_up * d_dn;
-      const double u = draw_uniform(ch, &a.st[c], c, a.rng, sweep, a.t, i, underrun);
-      const bool acc = u <= ratio;
-      if (!isfinite(ratio)) bad = true;
-      if (a.trace && threadIdx.x == 0) {
-        ch.tr_ratio[(size_t)a.t * N + i] = ratio;
-        ch.tr_acc[(size_t)a.t * N + i] = acc ? 1 : 0;
-      }
-      if (!acc) continue;
-      for (int e = threadIdx.x; e < 2 * kb; e += blockDim.x) {
-        const int sp = e / kb, q = e % kb;
-        xs[sp * K + q] = small[sp * K * K + q * K + jj];
-        double y = -small[sp * K * K + jj * K + q];
-        if (q == jj) y = y + 1.0;
-        ys[sp * K + q] = y;
-      }
-      if (threadIdx.x < 2) {
-        const double al = threadIdx.x == 0 ? ep - 1.0 : em - 1.0;
-        const double de = threadIdx.x == 0 ? d_up : d_dn;
-        coef[threadIdx.x] = al / de;
-        alpha[threadIdx.x * K + m] = al;
-      }
-      if (threadIdx.x == 0) {
-        aidx[m] = i;
-        fld[i] = (int8_t)(-s);
-      }
-      __syncthreads();
-      for (int e = threadIdx.x; e < 2 * kb * kb; e += blockDim.x) {
-        const int sp = e / (kb * kb), r = e % (kb * kb);
-        const int p = r / kb, q = r % kb;
-        small[sp * K * K + p * K + q] -= coef[sp] * (xs[sp * K + p] * ys[sp * K + q]);
-      }
-      __syncthreads();
-      ++m;
+    if (threadIdx.x < kb) {
+      const int i = b0 + threadIdx.x;
+      sb[threadIdx.x] = fld[i];
+      bool und = false;
+      ub[threadIdx.x] = draw_uniform(ch, &a.st[c], c, a.rng, sweep, a.t, i, und);
+      if (und) underrun = true;
     }
+    __syncthreads();
+    if (wave == 0) {
+      int m = 0;
+      for (int jj = 0; jj < kb; ++jj) {
+        const int s = sb[jj];
+        const double gu = small[jj * K + jj];
+        const double gd = small[K * K + jj * K + jj];
+        const double ep = s > 0 ? ch.e2p : ch.e2m;
+        const double em = s > 0 ? ch.e2m : ch.e2p;
+        const double d_up = 1.0 + (1.0 - gu) * (ep - 1.0);
+        const double d_dn = 1.0 + (1.0 - gd) * (em - 1.0);
+        const double ratio = d_up * d_dn;
+        const bool acc = ub[jj] <= ratio;
+        if (!isfinite(ratio)) bad = true;
+        if (a.trace && lane == 0) {
+          ch.tr_ratio[(size_t)a.t * N + b0 + jj] = ratio;
+          ch.tr_acc[(size_t)a.t * N + b0 + jj] = acc ? 1 : 0;
+        }
+        if (!acc) continue;
+        const double cfs[2] = {(ep - 1.0) / d_up, (em - 1.0) / d_dn};
+        if (lane < kb) {
+          for (int sp = 0; sp < 2; ++sp) {
+            const double xq = small[sp * K * K + lane * K + jj];
+            double yq = -small[sp * K * K + jj * K + lane];
+            if (lane == jj) yq = yq + 1.0;
+            xs[sp * K + lane] = xq;
+            ys[sp * K + lane] = yq;
+            Xsb[sp * K * K + lane * K + m] = cfs[sp] * xq;
+            Ysb[sp * K * K + lane * K + m] = yq;
+          }
+        }
+        if (lane == 0) {
+          cf[m] = cfs[0];
+          cf[K + m] = cfs[1];
+          aidx[m] = jj;
+          fld[b0 + jj] = (int8_t)(-s);
+        }
+        wave_sync();
+        if (lane < kb)
+          for (int sp = 0; sp < 2; ++sp) {
+            const double yl = ys[sp * K + lane];
+            double* col = small + sp * K * K + lane;
+            for (int p = 0; p < kb; ++p) col[p * K] -= cfs[sp] * (xs[sp * K + p] * yl);
+          }
+        wave_sync();
+        ++m;
+      }
+      // unit upper-triangular inverses per spin, lane j owns column j; results
+      // scattered onto the block rows: Mx[aidx[p]][j] = c_p ((I+UC)^-1)[p][j], My likewise
+      for (int sp = 0; sp < 2; ++sp) {
+        double* iu = small;          // [K][K]
+        double* il = small + K * K;  // [K][K]
+        double* Xs = Xsb + sp * K * K;
+        double* Ys = Ysb + sp * K * K;
+        const double* cs = cf + sp * K;
+        const int j = lane;
+        if (j < m) {
+          iu[j * K + j] = 1.0;
+          il[j * K + j] = 1.0;
+          for (int r = j - 1; r >= 0; --r) {
+            double su = 0.0, sl = 0.0;
+            for (int p = r + 1; p <= j; ++p) {
+              const int ap = aidx[p];
+              su += Ys[ap * K + r] * cs[p] * iu[p * K + j];
+              sl -= Xs[ap * K + r] * il[p * K + j];
+            }
+            iu[r * K + j] = -su;
+            il[r * K + j] = -sl;
+          }
+        }
+        wave_sync();
+        if (j < m) {
+          for (int q = 0; q < kb; ++q) {
+            Xs[q * K + j] = 0.0;
+            Ys[q * K + j] = 0.0;
+          }
+          for (int r = 0; r <= j; ++r) {
+            Xs[aidx[r] * K + j] = cs[r] * iu[r * K + j];
+            Ys[aidx[r] * K + j] = il[r * K + j];
+          }
+        }
+        wave_sync();
+      }
+      if (lane == 0) m_sh = m;
+    }
+    __syncthreads();
+    const int m = m_sh;
     if (m == 0) continue;
     nacc += m;
     const int K4 = (m + 3) & ~3;
     for (int sp = 0; sp < 2; ++sp) {
-      T* Gs = G + sp * n2;
-      // Gamma = diag(1/alpha) + I - G[A,A], augmented with I for Gauss-Jordan
-      for (int e = threadIdx.x; e < m * 2 * m; e += blockDim.x) {
-        const int p = e / (2 * m), q = e % (2 * m);
-        double v;
-        if (q < m) {
-          v = -(double)Gs[(size_t)aidx[p] * N + aidx[q]];
-          if (p == q) v += 1.0 + 1.0 / alpha[sp * K + p];
-        } else {
-          v = (q - m == p) ? 1.0 : 0.0;
-        }
-        gam[p * 2 * K + q] = v;
-      }
-      __syncthreads();
-      for (int j = 0; j < m; ++j) {
-        if (threadIdx.x == 0) {
-          int pv = j;
-          double best = fabs(gam[j * 2 * K + j]);
-          for (int r = j + 1; r < m; ++r) {
-            const double v = fabs(gam[r * 2 * K + j]);
-            if (v > best) { best = v; pv = r; }
-          }
-          piv_s = pv;
-        }
-        __syncthreads();
-        const int pv = piv_s;
-        if (pv != j)
-          for (int q = threadIdx.x; q < 2 * m; q += blockDim.x) {
-            const double t0 = gam[j * 2 * K + q];
-            gam[j * 2 * K + q] = gam[pv * 2 * K + q];
-            gam[pv * 2 * K + q] = t0;
-          }
-        __syncthreads();
-        const double ip = 1.0 / gam[j * 2 * K + j];
-        __syncthreads();
-        for (int q = threadIdx.x; q < 2 * m; q += blockDim.x) gam[j * 2 * K + q] *= ip;
-        __syncthreads();
-        for (int e = threadIdx.x; e < m * 2 * m; e += blockDim.x) {
-          const int r = e / (2 * m), q = e % (2 * m);
-          if (r != j) {
-            const double f = gam[r * 2 * K + j];
-            if (q != j) gam[r * 2 * K + q] -= f * gam[j * 2 * K + q];
-          }
-        }
-        __syncthreads();
-        for (int r = threadIdx.x; r < m; r += blockDim.x)
-          if (r != j) gam[r * 2 * K + j] = 0.0;
-        __syncthreads();
-      }
-      // P = -G[:,A] Gamma^-1  (Np x K4), Z = E_A - G[A,:]  (K4 x Np)
-      for (int e = threadIdx.x; e < Np * K4; e += blockDim.x) {
-        const int r = e / K4, q = e % K4;
-        double v = 0.0;
-        if (r < N && q < m)
-          for (int p = 0; p < m; ++p) v += (double)Gs[(size_t)r * N + aidx[p]] * gam[p * 2 * K + m + q];
-        P[(size_t)r * pld + q] = (T)(-v);
-      }
-      for (int e = threadIdx.x; e < K4 * Np; e += blockDim.x) {
-        const int p = e / Np, col = e % Np;
-        double v = 0.0;
-        if (p < m && col < N) {
-          v = -(double)Gs[(size_t)aidx[p] * N + col];
-          if (col == aidx[p]) v += 1.0;
-        }
-        Z[(size_t)p * Np + col] = (T)v;
-      }
-      __syncthreads();
-      rank_update<T>(Gs, N, Np, P, pld, Z, K4);
-      __syncthreads();
+      T* P = wsT + (size_t)sp * spT;
+      strip_products<T>(G + sp * n2, N, Np, b0, kb, m, K4, Xsb + sp * K * K, Ysb + sp * K * K, K, P, pld,
+                        P + (size_t)Np * pld);
     }
+    __syncthreads();
+    for (int sp = 0; sp < 2; ++sp) {
+      T* P = wsT + (size_t)sp * spT;
+      rank_update<T>(G + sp * n2, N, Np, P, pld, P + (size_t)Np * pld, K4);
+    }
+    __syncthreads();
   }
   slice_epilogue<T>(a, ch, c, l, G, nacc, underrun, bad, red);
 }
 
 size_t flip_delayed_shmem(int nmax, int k, int threads, int fp32) {
-  const int Np = (nmax + 15) & ~15;
-  const int K4 = (k + 3) & ~3;
-  const size_t esz = fp32 ? 4 : 8;
-  size_t head = sizeof(double) * (threads + 2 * k + 2 * (size_t)k * k) + sizeof(int) * ((k + 1) & ~1);
-  size_t serial = sizeof(double) * (2 * (size_t)k * k + 4 * k);
-  size_t update = esz * ((size_t)Np * (K4 + 1) + (size_t)K4 * Np);
-  return head + (serial > update ? serial : update);
+  (void)nmax;
+  (void)fp32;
+  return sizeof(double) * (threads + 6 * (size_t)k * k + 7 * (size_t)k) + sizeof(int) * 2 * k;
 }
 
-// Allow dynamic LDS beyond the default per-kernel limit (gfx950 has 160 KB per CU).
-int flip_prepare(size_t shmem) {
-  hipError_t e = hipSuccess;
-  const int b = (int)shmem;
-  hipError_t r;
-  r = hipFuncSetAttribute((const void*)k_flip_delayed<float>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-  if (r != hipSuccess) e = r;
-  r = hipFuncSetAttribute((const void*)k_flip_delayed<double>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-  if (r != hipSuccess) e = r;
-  r = hipFuncSetAttribute((const void*)k_flip_rank1<float>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-  if (r != hipSuccess) e = r;
-  r = hipFuncSetAttribute((const void*)k_flip_rank1<double>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-  if (r != hipSuccess) e = r;
-  return (int)e;
+// Bytes of the per-chain delayed-update workspace (P and Z of both spins).
+size_t flip_delayed_workspace(int n, int k, int fp32) {
+  const size_t Np = (n + 15) & ~15;
+  const size_t K16 = (k + 15) & ~15;
+  return 2 * (Np * (K16 + 1) + K16 * Np) * (fp32 ? 4 : 8);
 }
 
 void launch_flip(int fp32, int delayed, const FlipArgs& a, int nchains, int threads, size_t shmem,

@@ -1,0 +1,108 @@
+// Bounds check of the flip kernels in a CPU build (AddressSanitizer): every
+// device buffer is a separate exact-size allocation and the dynamic LDS is an
+// exact-size heap block, so any out-of-range access aborts with a report.
+// usage: flip_harness N L delay threads fp32 nchains t measure
+#include "hip_cpu_shim.h"
+#include "../../latticeqmc_amd/csrc/flip.hip"
+
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+
+using namespace lqmc;
+
+template <typename T>
+static T* dalloc(size_t n) { return static_cast<T*>(std::calloc(n ? n : 1, sizeof(T))); }
+
+int main(int argc, char** argv) {
+  if (argc < 9) { std::fprintf(stderr, "usage: N L delay threads fp32 nchains t measure\n"); return 2; }
+  const int N = atoi(argv[1]), L = atoi(argv[2]), K = atoi(argv[3]), threads = atoi(argv[4]);
+  const int fp32 = atoi(argv[5]), nch = atoi(argv[6]), t = atoi(argv[7]), measure = atoi(argv[8]);
+  const size_t n2 = (size_t)N * N;
+  std::mt19937_64 rng(1);
+  std::uniform_real_distribution<double> U(-0.3, 0.3);
+  const double gscale = argc > 9 ? atof(argv[9]) : 1.0;  // off-diagonal scale (larger -> more rejections)
+  std::vector<ChainDev> chains(nch);
+  std::vector<ChainState> st(nch);
+  for (int c = 0; c < nch; ++c) {
+    ChainDev& d = chains[c];
+    std::memset(&d, 0, sizeof d);
+    d.N = N; d.L = L; d.mode = 0;
+    d.lam = 0.6; d.e1m = std::exp(-0.6); d.e1p = std::exp(0.6); d.e2p = std::exp(1.2); d.e2m = std::exp(-1.2);
+    d.seed = 1234 + c;
+    d.field = dalloc<int8_t>((size_t)N * L);
+    for (size_t e = 0; e < (size_t)N * L; ++e) d.field[e] = (rng() & 1) ? 1 : -1;
+    d.ev = dalloc<double>(2 * (size_t)N * L);
+    d.evi = dalloc<double>(2 * (size_t)N * L);
+    if (fp32) {
+      float* g = dalloc<float>(2 * n2);
+      for (size_t e = 0; e < 2 * n2; ++e) g[e] = (float)(gscale * U(rng) / N + ((e % n2) / N == (e % n2) % N ? 0.5 : 0.0));
+      d.G = g;
+    } else {
+      double* g = dalloc<double>(2 * n2);
+      for (size_t e = 0; e < 2 * n2; ++e) g[e] = gscale * U(rng) / N + ((e % n2) / N == (e % n2) % N ? 0.5 : 0.0);
+      d.G = g;
+    }
+    d.Gacc = dalloc<double>(2 * n2);
+    d.Glast = dalloc<double>(2 * n2);
+    d.hop = dalloc<double>(n2);
+    d.uni = nullptr;
+    d.tr_ratio = dalloc<double>((size_t)N * L);
+    d.tr_acc = dalloc<uint8_t>((size_t)N * L);
+    d.obs_cap = 8;
+    d.obs = dalloc<double>(3 * 8);
+    if (K > 1) {
+      d.dws = dalloc<char>(flip_delayed_workspace(N, K, fp32));
+    }
+  }
+  std::memset(st.data(), 0, sizeof(ChainState) * nch);
+  FlipArgs a{chains.data(), st.data(), t, measure, 1, 1, K, 0};
+  const size_t lds = K > 1 ? flip_delayed_shmem(N, K, threads, fp32) : sizeof(double) * (4 * (size_t)N + threads);
+  std::printf("N=%d L=%d K=%d threads=%d fp32=%d chains=%d t=%d lds=%zu\n", N, L, K, threads, fp32, nch, t, lds);
+  std::fflush(stdout);
+  if (K > 1) {
+    if (fp32) emu_launch(nch, threads, lds, [&] { k_flip_delayed<float>(a); });
+    else emu_launch(nch, threads, lds, [&] { k_flip_delayed<double>(a); });
+  } else {
+    if (fp32) emu_launch(nch, threads, lds, [&] { k_flip_rank1<float>(a); });
+    else emu_launch(nch, threads, lds, [&] { k_flip_rank1<double>(a); });
+  }
+  unsigned long long acc = 0;
+  for (auto& s : st) acc += s.accepted;
+  // cross-check against the rank-1 kernel on a copy of the initial state (same uniforms)
+  if (K > 1 && argc > 9) {
+    std::vector<ChainDev> ref(nch);
+    std::vector<ChainState> st2(nch);
+    std::memset(st2.data(), 0, sizeof(ChainState) * nch);
+    std::mt19937_64 rng2(1);
+    double worst = 0.0;
+    int fdiff = 0;
+    for (int c = 0; c < nch; ++c) {
+      ref[c] = chains[c];
+      ChainDev& d = ref[c];
+      d.field = dalloc<int8_t>((size_t)N * L);
+      for (size_t e = 0; e < (size_t)N * L; ++e) d.field[e] = (rng2() & 1) ? 1 : -1;
+      double* g = dalloc<double>(2 * n2);
+      for (size_t e = 0; e < 2 * n2; ++e) g[e] = gscale * U(rng2) / N + ((e % n2) / N == (e % n2) % N ? 0.5 : 0.0);
+      d.G = g;
+      d.Gacc = dalloc<double>(2 * n2); d.Glast = dalloc<double>(2 * n2);
+      d.ev = dalloc<double>(2 * (size_t)N * L); d.evi = dalloc<double>(2 * (size_t)N * L);
+      d.tr_ratio = dalloc<double>((size_t)N * L); d.tr_acc = dalloc<uint8_t>((size_t)N * L);
+      d.obs = dalloc<double>(3 * 8);
+    }
+    FlipArgs b{ref.data(), st2.data(), t, measure, 1, 1, 1, 0};
+    emu_launch(nch, threads, sizeof(double) * (4 * (size_t)N + threads), [&] { k_flip_rank1<double>(b); });
+    for (int c = 0; c < nch; ++c) {
+      const double* g1 = static_cast<double*>(ref[c].G);
+      for (size_t e = 0; e < 2 * n2; ++e) {
+        const double g0 = fp32 ? (double)static_cast<float*>(chains[c].G)[e] : static_cast<double*>(chains[c].G)[e];
+        worst = std::max(worst, std::fabs(g0 - g1[e]));
+      }
+      for (size_t e = 0; e < (size_t)N * L; ++e) fdiff += chains[c].field[e] != ref[c].field[e];
+    }
+    std::printf("vs rank-1: max|dG| %.3e, field differences %d\n", worst, fdiff);
+    if (fdiff || worst > (fp32 ? 1e-4 : 1e-11)) { std::printf("FAIL\n"); return 1; }
+  }
+  std::printf("ok: accepted %llu of %llu, flags %u\n", acc, (unsigned long long)nch * N, st[0].flags);
+  return 0;
+}

@@ -1,0 +1,112 @@
+// CPU emulation of the small HIP subset the kernels use, for bounds checking
+// under AddressSanitizer: one OS thread per GPU thread, __syncthreads() as a
+// block barrier, the 16x16x4 matrix-core ops emulated across the 64 threads of
+// a wavefront with the lane layout measured on gfx950 (inputs: lane l holds
+// A[l%16][l/16] and B[l/16][l%16]; outputs: column l%16, row l/16 + 4r for
+// FP64 and 4*(l/16) + r for FP32).  Blocks run one after another.
+#pragma once
+#include <barrier>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#define LQMC_CPU_EMU 1
+#define __global__
+#define __device__
+#define __host__
+#define __launch_bounds__(x)
+#define __shared__ static
+#define __restrict__
+
+struct emu_dim3 { unsigned x = 1, y = 1, z = 1; };
+inline thread_local emu_dim3 threadIdx, blockIdx;
+inline emu_dim3 blockDim, gridDim;
+inline char* emu_lds_buf = nullptr;
+inline std::barrier<>* emu_block_barrier = nullptr;
+#define LQMC_DYNAMIC_LDS(type, name) type* name = reinterpret_cast<type*>(emu_lds_buf)
+inline void __syncthreads() { emu_block_barrier->arrive_and_wait(); }
+
+using std::fabs; using std::fmax; using std::fmin; using std::isfinite; using std::max; using std::min; using std::sqrt;
+
+typedef int hipError_t;
+typedef void* hipStream_t;
+constexpr int hipSuccess = 0;
+constexpr int hipFuncAttributeMaxDynamicSharedMemorySize = 8;
+inline int hipFuncSetAttribute(const void*, int, int) { return 0; }
+#define hipLaunchKernelGGL(...) ((void)0)
+
+inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline unsigned atomicOr(unsigned* p, unsigned v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
+inline unsigned long long atomicMax(unsigned long long* p, unsigned long long v) {
+  unsigned long long cur = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+  while (cur < v && !__atomic_compare_exchange_n(p, &cur, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {}
+  return cur;
+}
+inline uint32_t __umulhi(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
+inline long long __double_as_longlong(double d) { long long r; std::memcpy(&r, &d, 8); return r; }
+
+struct EmuWave { double a[64], b[64]; std::barrier<>* bar; };
+inline EmuWave* emu_waves = nullptr;
+
+template <typename T, typename V>
+inline V emu_mfma(T a, T b, V c, bool fp64) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  EmuWave& s = emu_waves[w];
+  s.a[lane] = a;
+  s.b[lane] = b;
+  s.bar->arrive_and_wait();
+  V d;
+  for (int r = 0; r < 4; ++r) {
+    const int row = fp64 ? (lane >> 4) + 4 * r : ((lane >> 4) << 2) + r;
+    const int col = lane & 15;
+    double acc = 0.0;
+    for (int k = 0; k < 4; ++k) acc += s.a[row + 16 * k] * s.b[col + 16 * k];
+    d[r] = (T)((double)c[r] + acc);
+  }
+  s.bar->arrive_and_wait();
+  return d;
+}
+#define __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, x, y, z) emu_mfma<double>((a), (b), (c), true)
+#define __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, x, y, z) emu_mfma<float>((a), (b), (c), false)
+
+// Wavefront-level ordering: all 64 lanes of the calling wave meet.
+inline void emu_wave_sync() { emu_waves[threadIdx.x >> 6].bar->arrive_and_wait(); }
+#define __builtin_amdgcn_wave_barrier() emu_wave_sync()
+#define __builtin_amdgcn_fence(order, scope) ((void)0)
+
+// Cross-lane exchange within a wavefront.
+inline double __shfl_xor(double v, int mask, int width = 64) {
+  (void)width;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  EmuWave& s = emu_waves[w];
+  s.a[lane] = v;
+  s.bar->arrive_and_wait();
+  const double r = s.a[lane ^ mask];
+  s.bar->arrive_and_wait();
+  return r;
+}
+
+// Run fn() as a block of `threads` threads for every block index in [0, blocks).
+template <typename F>
+inline void emu_launch(int blocks, int threads, size_t lds_bytes, F fn) {
+  blockDim.x = threads;
+  gridDim.x = blocks;
+  std::vector<EmuWave> waves((threads + 63) / 64);
+  std::vector<std::barrier<>*> wb;
+  for (auto& w : waves) { w.bar = new std::barrier<>(64); wb.push_back(w.bar); }
+  emu_waves = waves.data();
+  std::barrier<> bar(threads);
+  emu_block_barrier = &bar;
+  for (int b = 0; b < blocks; ++b) {
+    emu_lds_buf = static_cast<char*>(std::malloc(lds_bytes ? lds_bytes : 1));
+    std::memset(emu_lds_buf, 0xA5, lds_bytes);
+    std::vector<std::thread> ts;
+    for (int t = 0; t < threads; ++t)
+      ts.emplace_back([&, t, b] { threadIdx.x = t; blockIdx.x = b; fn(); });
+    for (auto& t : ts) t.join();
+    std::free(emu_lds_buf);
+  }
+  for (auto* p : wb) delete p;
+}

@@ -142,3 +142,34 @@ def test_batched_uneven_equals_alone():
         assert alone.stats(0)["accepted"] == batched.stats(i)["accepted"]
         alone.close()
     batched.close()
+
+
+def test_dqmc_fp32_delayed_tracks_fp64():
+    _say("[dqmc fp32 delayed]")
+    side, L, beta, n_stab = 4, 16, 2.0, 4
+    k, f = square_case(side, L, beta, 4.0, 2)
+    n = side * side
+    o = dqmc.DQMCOracle(k, beta, L, 4.0, f, n_stab=n_stab)
+    eng = SweepEngine([spec_from(k, beta, L, 4.0, f)], mode="dqmc", precision="fp32", delay=8,
+                      n_stab=n_stab, rng="replay", trace=True)
+    rs = np.random.RandomState(7)
+    uni = rs.rand(L * n)
+    res = o.sweep(uni)
+    eng.set_uniforms(0, uni)
+    eng.sweep(1, measure=True)
+    eng.sync()
+    ratio, acc = eng.trace(0)
+    # FP32 G deviates ~1e-6 from FP64: accept bits agree away from near-ties
+    ok, ties = masked_equal(acc, res["accept"], res["ratio"], uni, tol=1e-3)
+    assert ok
+    st = eng.stats(0)
+    assert 0 < st["max_err"] < 1e-3 and st["flags"] == 0
+    eng.close()
+
+
+def test_engine_refuses_oversized_lds():
+    _say("[lds limit]")
+    from latticeqmc_amd._native import LqmcError
+    k, f = square_case(16, 8, 1.0, 4.0, 0)
+    with pytest.raises(LqmcError, match="LDS"):
+        SweepEngine([spec_from(k, 1.0, 8, 4.0, f)], mode="dqmc", delay=64, n_stab=4, rng="philox")
