@@ -165,6 +165,43 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
     __syncthreads();
     for (int q = 0; q < w; ++q) {
       const int j = j0 + q;
+      // panel-local pivoting: the remaining panel column of largest updated norm
+      {
+        const int lane = tid & 63, wave = tid >> 6;
+        for (int q2 = q + wave; q2 < w; q2 += QR_WAVES) {
+          double s2 = 0.0;
+          for (int r = j + lane; r < N; r += 64) {
+            const double x = pnl[r * nb + q2];
+            s2 += x * x;
+          }
+          s2 = qr_wave_sum(s2);
+          if (lane == 0) Wt[q2] = s2;
+        }
+        __syncthreads();
+        int p = q;
+        double best = Wt[q];
+        for (int q2 = q + 1; q2 < w; ++q2)
+          if (Wt[q2] > best) { best = Wt[q2]; p = q2; }
+        if (p != q) {
+          for (int r = j0 + tid; r < N; r += QR_THREADS) {
+            const double t0 = pnl[r * nb + q];
+            pnl[r * nb + q] = pnl[r * nb + p];
+            pnl[r * nb + p] = t0;
+          }
+          // R entries of rows above the panel live in A
+          for (int r = tid; r < j0; r += QR_THREADS) {
+            const double t0 = A[(size_t)r * N + j0 + q];
+            A[(size_t)r * N + j0 + q] = A[(size_t)r * N + j0 + p];
+            A[(size_t)r * N + j0 + p] = t0;
+          }
+          if (tid == 0) {
+            const int tp = perm[j0 + q];
+            perm[j0 + q] = perm[j0 + p];
+            perm[j0 + p] = tp;
+          }
+        }
+        __syncthreads();
+      }
       double s = 0.0;
       for (int r = j + tid; r < N; r += QR_THREADS) {
         const double x = pnl[r * nb + q];
