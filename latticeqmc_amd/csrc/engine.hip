@@ -39,6 +39,7 @@ void launch_flip(int fp32, int delayed, const FlipArgs& a, int nchains, int thre
                  hipStream_t s);
 size_t flip_delayed_shmem(int nmax, int k, int threads, int fp32);
 size_t flip_delayed_workspace(int n, int k, int fp32);
+size_t qr_blocked_shmem(int nmax);
 }  // namespace lqmc
 
 using namespace lqmc;
@@ -597,7 +598,8 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
   if (o.precision != LQMC_FP64 && o.precision != LQMC_FP32) return fail(LQMC_EINVAL, "unknown precision");
   if (o.rng != LQMC_RNG_REPLAY && o.rng != LQMC_RNG_PHILOX) return fail(LQMC_EINVAL, "unknown rng");
   if (o.delay < 1 || o.delay > 64) return fail(LQMC_EINVAL, "delay must be in 1..64");
-  if (o.delay > 1 && o.delay % 4 != 0) return fail(LQMC_EINVAL, "delay must be 1 or a multiple of 4");
+  if (o.delay != 1 && o.delay != 4 && o.delay != 8 && o.delay != 16 && o.delay != 32)
+    return fail(LQMC_EINVAL, "delay must be 1 (rank-1) or one of 4, 8, 16, 32 (delayed blocks)");
   if (o.n_stab < 1) return fail(LQMC_EINVAL, "n_stab must be >= 1");
   if (o.mode == LQMC_MODE_REFERENCE && (o.precision != LQMC_FP64 || o.delay != 1))
     return fail(LQMC_EINVAL, "reference mode runs the FP64 rank-1 sweep only");
@@ -635,6 +637,14 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
   if ((int)x->flip_shmem > maxsh) {
     char buf[160];
     snprintf(buf, sizeof buf, "flip kernel needs %zu B of LDS (> %d); lower the delay", x->flip_shmem, maxsh);
+    delete x;
+    return fail(LQMC_EINVAL, buf);
+  }
+  // The blocked QR keeps one column panel of the largest chain in LDS.
+  if (o.mode == LQMC_MODE_DQMC && qr_blocked_shmem(x->nmax) > (size_t)maxsh) {
+    char buf[160];
+    snprintf(buf, sizeof buf, "QR kernel needs %zu B of LDS (> %d) for N = %d; reduce the lattice size",
+             qr_blocked_shmem(x->nmax), maxsh, x->nmax);
     delete x;
     return fail(LQMC_EINVAL, buf);
   }

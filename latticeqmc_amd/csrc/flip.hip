@@ -335,16 +335,16 @@ __device__ void strip_products(const T* G, int N, int Np, int b0, int kb, int m,
   }
 }
 
-template <typename T>
+template <typename T, int K>
 __global__ __launch_bounds__(1024) void k_flip_delayed(FlipArgs a) {
+  static_assert(K >= 4 && K <= 32 && K % 4 == 0, "block size");
   const int c = blockIdx.x;
   const ChainDev ch = a.chains[c];
   if (a.t >= ch.L) return;
   const int N = ch.N;
   const int l = ch.L - 1 - a.t;
-  const int K = a.delay;
   const int Np = (N + 15) & ~15;
-  const int K16 = (K + 15) & ~15;
+  constexpr int K16 = (K + 15) & ~15;
   const int pld = K16 + 1;
   const size_t n2 = (size_t)N * N;
   T* __restrict__ G = static_cast<T*>(ch.G);
@@ -355,12 +355,9 @@ __global__ __launch_bounds__(1024) void k_flip_delayed(FlipArgs a) {
 
   LQMC_DYNAMIC_LDS(double, sm);
   double* red = sm;                        // blockDim (epilogue reductions)
-  double* small = red + blockDim.x;        // [2][K][K]; later the triangular-inverse scratch
-  double* Xsb = small + 2 * K * K;         // [2][K][K]: x_j block parts, later Mx scattered
+  double* Xsb = red + blockDim.x;          // [2][K rows][K accepts]: x_j block parts, later Mx scattered
   double* Ysb = Xsb + 2 * K * K;           // [2][K][K]: y_j block parts, later My scattered
-  double* xs = Ysb + 2 * K * K;            // [2][K]
-  double* ys = xs + 2 * K;                 // [2][K]
-  double* cf = ys + 2 * K;                 // [2][K] coefficients of the accepts
+  double* cf = Ysb + 2 * K * K;            // [2][K] coefficients of the accepts
   double* ub = cf + 2 * K;                 // [K] uniforms of the block
   int* aidx = reinterpret_cast<int*>(ub + K);  // [K] block-local site of each accept
   int* sb = aidx + K;                          // [K] field entries of the block
@@ -373,11 +370,6 @@ __global__ __launch_bounds__(1024) void k_flip_delayed(FlipArgs a) {
 
   for (int b0 = 0; b0 < N; b0 += K) {
     const int kb = min(K, N - b0);
-    for (int e = threadIdx.x; e < 2 * kb * kb; e += blockDim.x) {
-      const int sp = e / (kb * kb), r = e % (kb * kb);
-      const int p = r / kb, q = r % kb;
-      small[sp * K * K + p * K + q] = (double)G[sp * n2 + (size_t)(b0 + p) * N + b0 + q];
-    }
     if (threadIdx.x < kb) {
       const int i = b0 + threadIdx.x;
       sb[threadIdx.x] = fld[i];
@@ -387,86 +379,54 @@ __global__ __launch_bounds__(1024) void k_flip_delayed(FlipArgs a) {
     }
     __syncthreads();
     if (wave == 0) {
+      // lane (sp, q) = (lane / K, lane % K) holds column q of spin sp's k x k block
+      const int sp = lane / K < 2 ? lane / K : 1;
+      const int q = lane % K;
+      const bool live = lane < 2 * K && q < kb;
+      double col[K];
+#pragma unroll
+      for (int p = 0; p < K; ++p)
+        col[p] = (live && p < kb) ? (double)G[sp * n2 + (size_t)(b0 + p) * N + b0 + q] : 0.0;
       int m = 0;
-      for (int jj = 0; jj < kb; ++jj) {
-        const int s = sb[jj];
-        const double gu = small[jj * K + jj];
-        const double gd = small[K * K + jj * K + jj];
-        const double ep = s > 0 ? ch.e2p : ch.e2m;
-        const double em = s > 0 ? ch.e2m : ch.e2p;
-        const double d_up = 1.0 + (1.0 - gu) * (ep - 1.0);
-        const double d_dn = 1.0 + (1.0 - gd) * (em - 1.0);
-        const double ratio = d_up * d_dn;
-        const bool acc = ub[jj] <= ratio;
-        if (!isfinite(ratio)) bad = true;
-        if (a.trace && lane == 0) {
-          ch.tr_ratio[(size_t)a.t * N + b0 + jj] = ratio;
-          ch.tr_acc[(size_t)a.t * N + b0 + jj] = acc ? 1 : 0;
-        }
-        if (!acc) continue;
-        const double cfs[2] = {(ep - 1.0) / d_up, (em - 1.0) / d_dn};
-        if (lane < kb) {
-          for (int sp = 0; sp < 2; ++sp) {
-            const double xq = small[sp * K * K + lane * K + jj];
-            double yq = -small[sp * K * K + jj * K + lane];
-            if (lane == jj) yq = yq + 1.0;
-            xs[sp * K + lane] = xq;
-            ys[sp * K + lane] = yq;
-            Xsb[sp * K * K + lane * K + m] = cfs[sp] * xq;
-            Ysb[sp * K * K + lane * K + m] = yq;
+#pragma unroll
+      for (int jj = 0; jj < K; ++jj) {
+        if (jj < kb) {
+          const double gu = __shfl(col[jj], jj, 64);
+          const double gd = __shfl(col[jj], K + jj, 64);
+          const int s = sb[jj];
+          const double ep = s > 0 ? ch.e2p : ch.e2m;
+          const double em = s > 0 ? ch.e2m : ch.e2p;
+          const double d_up = 1.0 + (1.0 - gu) * (ep - 1.0);
+          const double d_dn = 1.0 + (1.0 - gd) * (em - 1.0);
+          const double ratio = d_up * d_dn;
+          const bool acc = ub[jj] <= ratio;
+          if (!isfinite(ratio)) bad = true;
+          if (a.trace && lane == 0) {
+            ch.tr_ratio[(size_t)a.t * N + b0 + jj] = ratio;
+            ch.tr_acc[(size_t)a.t * N + b0 + jj] = acc ? 1 : 0;
           }
-        }
-        if (lane == 0) {
-          cf[m] = cfs[0];
-          cf[K + m] = cfs[1];
-          aidx[m] = jj;
-          fld[b0 + jj] = (int8_t)(-s);
-        }
-        wave_sync();
-        if (lane < kb)
-          for (int sp = 0; sp < 2; ++sp) {
-            const double yl = ys[sp * K + lane];
-            double* col = small + sp * K * K + lane;
-            for (int p = 0; p < kb; ++p) col[p * K] -= cfs[sp] * (xs[sp * K + p] * yl);
-          }
-        wave_sync();
-        ++m;
-      }
-      // unit upper-triangular inverses per spin, lane j owns column j; results
-      // scattered onto the block rows: Mx[aidx[p]][j] = c_p ((I+UC)^-1)[p][j], My likewise
-      for (int sp = 0; sp < 2; ++sp) {
-        double* iu = small;          // [K][K]
-        double* il = small + K * K;  // [K][K]
-        double* Xs = Xsb + sp * K * K;
-        double* Ys = Ysb + sp * K * K;
-        const double* cs = cf + sp * K;
-        const int j = lane;
-        if (j < m) {
-          iu[j * K + j] = 1.0;
-          il[j * K + j] = 1.0;
-          for (int r = j - 1; r >= 0; --r) {
-            double su = 0.0, sl = 0.0;
-            for (int p = r + 1; p <= j; ++p) {
-              const int ap = aidx[p];
-              su += Ys[ap * K + r] * cs[p] * iu[p * K + j];
-              sl -= Xs[ap * K + r] * il[p * K + j];
+          if (acc) {
+            const double cu = (ep - 1.0) / d_up, cd = (em - 1.0) / d_dn;
+            const double cfs = sp == 0 ? cu : cd;
+            const int src = sp * K + jj;  // lane holding x = column jj of my spin
+            double yq = -col[jj];
+            if (q == jj) yq = yq + 1.0;
+#pragma unroll
+            for (int p = 0; p < K; ++p) {
+              const double xp = __shfl(col[p], src, 64);
+              if (lane == src && p < kb) Xsb[sp * K * K + p * K + m] = cfs * xp;
+              if (live && p < kb) col[p] -= cfs * (xp * yq);
             }
-            iu[r * K + j] = -su;
-            il[r * K + j] = -sl;
+            if (live) Ysb[sp * K * K + q * K + m] = yq;
+            if (lane == 0) {
+              cf[m] = cu;
+              cf[K + m] = cd;
+              aidx[m] = jj;
+              fld[b0 + jj] = (int8_t)(-s);
+            }
+            ++m;
           }
         }
-        wave_sync();
-        if (j < m) {
-          for (int q = 0; q < kb; ++q) {
-            Xs[q * K + j] = 0.0;
-            Ys[q * K + j] = 0.0;
-          }
-          for (int r = 0; r <= j; ++r) {
-            Xs[aidx[r] * K + j] = cs[r] * iu[r * K + j];
-            Ys[aidx[r] * K + j] = il[r * K + j];
-          }
-        }
-        wave_sync();
       }
       if (lane == 0) m_sh = m;
     }
@@ -474,6 +434,43 @@ __global__ __launch_bounds__(1024) void k_flip_delayed(FlipArgs a) {
     const int m = m_sh;
     if (m == 0) continue;
     nacc += m;
+    // unit upper-triangular inverses, one per wave: waves 0/1 spin 0 (I+UC)^-1 / (I-L)^-1,
+    // waves 2/3 spin 1; lane j keeps column j in registers
+    double inv[K];
+    const int isp = wave >> 1;
+    const bool lower = (wave & 1) != 0;
+    const int j = lane;
+    if (wave < 4) {
+      const double* Xs = Xsb + isp * K * K;
+      const double* Ys = Ysb + isp * K * K;
+      const double* cs = cf + isp * K;
+#pragma unroll
+      for (int r = K - 1; r >= 0; --r) {
+        double v = 0.0;
+        if (r == j) v = 1.0;
+        else if (r < j && j < m) {
+#pragma unroll
+          for (int p = r + 1; p < K; ++p)
+            if (p <= j) {
+              const int ap = aidx[p];
+              const double t = lower ? -Xs[ap * K + r] : Ys[ap * K + r] * cs[p];
+              v -= t * inv[p];
+            }
+        }
+        inv[r] = v;
+      }
+    }
+    __syncthreads();
+    if (wave < 4 && j < m) {
+      // scatter onto the block rows: Mx[aidx[r]][j] = c_r inv[r][j], My[aidx[r]][j] = inv[r][j]
+      double* out = (lower ? Ysb : Xsb) + isp * K * K;
+      const double* cs = cf + isp * K;
+      for (int qq = 0; qq < kb; ++qq) out[qq * K + j] = 0.0;
+#pragma unroll
+      for (int r = 0; r < K; ++r)
+        if (r <= j) out[aidx[r] * K + j] = lower ? inv[r] : cs[r] * inv[r];
+    }
+    __syncthreads();
     const int K4 = (m + 3) & ~3;
     for (int sp = 0; sp < 2; ++sp) {
       T* P = wsT + (size_t)sp * spT;
@@ -493,7 +490,7 @@ __global__ __launch_bounds__(1024) void k_flip_delayed(FlipArgs a) {
 size_t flip_delayed_shmem(int nmax, int k, int threads, int fp32) {
   (void)nmax;
   (void)fp32;
-  return sizeof(double) * (threads + 6 * (size_t)k * k + 7 * (size_t)k) + sizeof(int) * 2 * k;
+  return sizeof(double) * (threads + 4 * (size_t)k * k + 3 * (size_t)k) + sizeof(int) * 2 * k;
 }
 
 // Bytes of the per-chain delayed-update workspace (P and Z of both spins).
@@ -506,8 +503,19 @@ size_t flip_delayed_workspace(int n, int k, int fp32) {
 void launch_flip(int fp32, int delayed, const FlipArgs& a, int nchains, int threads, size_t shmem,
                  hipStream_t s) {
   if (delayed) {
-    if (fp32) hipLaunchKernelGGL(k_flip_delayed<float>, dim3(nchains), dim3(threads), shmem, s, a);
-    else hipLaunchKernelGGL(k_flip_delayed<double>, dim3(nchains), dim3(threads), shmem, s, a);
+#define LQMC_DELAYED(KK)                                                                               \
+  case KK:                                                                                             \
+    if (fp32) hipLaunchKernelGGL((k_flip_delayed<float, KK>), dim3(nchains), dim3(threads), shmem, s, a); \
+    else hipLaunchKernelGGL((k_flip_delayed<double, KK>), dim3(nchains), dim3(threads), shmem, s, a);    \
+    break;
+    switch (a.delay) {
+      LQMC_DELAYED(4)
+      LQMC_DELAYED(8)
+      LQMC_DELAYED(16)
+      LQMC_DELAYED(32)
+      default: break;
+    }
+#undef LQMC_DELAYED
   } else {
     if (fp32) hipLaunchKernelGGL(k_flip_rank1<float>, dim3(nchains), dim3(threads), shmem, s, a);
     else hipLaunchKernelGGL(k_flip_rank1<double>, dim3(nchains), dim3(threads), shmem, s, a);

@@ -63,51 +63,152 @@ __device__ void qr_build_t(const double* pnl, int nb, int r0, int N, int w, cons
   __syncthreads();
 }
 
-// X[r0:, c_begin:] <- (I - V T' V^T) X[r0:, c_begin:], T' = T^T (trans) or T.
+typedef double qr_v4d __attribute__((ext_vector_type(4)));
+
+__device__ inline qr_v4d qr_mma(double a, double b, qr_v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// X[r0:N, c_begin:N] <- (I - V T' V^T) X[r0:N, c_begin:N] on the FP64 matrix cores,
+// V = pnl (explicit reflectors, rows r0..N-1, w <= 16 columns), T' = T^T (trans) or T.
+// One wavefront per 16-column tile:  W = V^T X (K = rows), W' = T' W, X -= V W'.
+// In the 16x16x4 FP64 layout the accumulator element v of lane l is row l/16 + 4v,
+// column l%16 -- exactly the B operand lane l supplies for k-step v -- so W and W'
+// chain through registers.  Ends with a workgroup barrier.
 __device__ void qr_apply_block(double* X, int N, int r0, int c_begin, const double* pnl, int nb, int w,
-                               const double* Tm, bool trans, double* Wt) {
+                               const double* Tm, bool trans) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int cc = c_begin; cc < N; cc += 64) {
-    const int c = cc + lane;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    if (c < N)
-      for (int r = r0; r < N; ++r) {
-        const double x = X[(size_t)r * N + c];
+  const int li = lane & 15, lk = lane >> 4;
+  const int nct = (N - c_begin + 15) >> 4;
+  for (int ct = wave; ct < nct; ct += QR_WAVES) {
+    const int c = c_begin + ct * 16 + li;
+    qr_v4d wt = {0.0, 0.0, 0.0, 0.0};
+    for (int kk = r0; kk < N; kk += 4) {
+      const int r = kk + lk;
+      const double a = (r < N && li < w) ? pnl[r * nb + li] : 0.0;
+      const double b = (r < N && c < N) ? X[(size_t)r * N + c] : 0.0;
+      wt = qr_mma(a, b, wt);
+    }
+    qr_v4d w2 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const int q = wave + QR_WAVES * k;
-          if (q < w) acc[k] += pnl[r * nb + q] * x;
-        }
+    for (int s = 0; s < 4; ++s) {
+      const int p = 4 * s + lk;
+      const double a = (li < w && p < w) ? (trans ? Tm[p * nb + li] : Tm[li * nb + p]) : 0.0;
+      w2 = qr_mma(a, wt[s], w2);
+    }
+    for (int rt = r0; rt < N; rt += 16) {
+      qr_v4d acc;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = rt + lk + 4 * v;
+        acc[v] = (row < N && c < N) ? X[(size_t)row * N + c] : 0.0;
+      }
+      const int ra = rt + li;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int q = 4 * s + lk;
+        const double a = (ra < N && q < w) ? -pnl[ra * nb + q] : 0.0;
+        acc = qr_mma(a, w2[s], acc);
       }
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int q = wave + QR_WAVES * k;
-      if (q < w) Wt[q * 64 + lane] = acc[k];
-    }
-    __syncthreads();
-    double tw[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int q = wave + QR_WAVES * k;
-      double s = 0.0;
-      if (q < w)
-        for (int p = 0; p < w; ++p) s += (trans ? Tm[p * nb + q] : Tm[q * nb + p]) * Wt[p * 64 + lane];
-      tw[k] = s;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int q = wave + QR_WAVES * k;
-      if (q < w) Wt[q * 64 + lane] = tw[k];
-    }
-    __syncthreads();
-    if (c < N)
-      for (int r = r0 + wave; r < N; r += QR_WAVES) {
-        double s = 0.0;
-        for (int q = 0; q < w; ++q) s += pnl[r * nb + q] * Wt[q * 64 + lane];
-        X[(size_t)r * N + c] -= s;
+      for (int v = 0; v < 4; ++v) {
+        const int row = rt + lk + 4 * v;
+        if (row < N && c < N) X[(size_t)row * N + c] = acc[v];
       }
-    __syncthreads();
+    }
+  }
+  __syncthreads();
+}
+
+// Orders LDS traffic between the lanes of one wavefront.
+__device__ inline void qr_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Householder factorisation of the panel (rows j0..N-1, w <= NB columns, in LDS) by one
+// wavefront, with panel-local column pivoting: at step q the remaining panel column of
+// largest norm over rows j..N-1 is swapped into place (with its R entries above the
+// panel in A and its pivot index).  The norms of all panel columns and the reflector's
+// dot products with them are accumulated in one pass each and reduced across lanes
+// together.  On return pnl holds R (on/above the diagonal) and the reflectors below it.
+template <int NB>
+__device__ void qr_panel_wave(double* pnl, double* A, int N, int j0, int w, int* perm, double* taus,
+                              bool& zero_pivot) {
+  const int lane = threadIdx.x & 63;
+  for (int q = 0; q < w; ++q) {
+    const int j = j0 + q;
+    double nrm[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) nrm[c] = 0.0;
+    for (int r = j + lane; r < N; r += 64) {
+      const double* row = pnl + r * NB;
+#pragma unroll
+      for (int c = 0; c < NB; ++c) nrm[c] += row[c] * row[c];
+    }
+#pragma unroll
+    for (int c = 0; c < NB; ++c) nrm[c] = qr_wave_sum(nrm[c]);
+    int p = q;
+    double best = -1.0;
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+      if (c >= q && c < w && nrm[c] > best) { best = nrm[c]; p = c; }
+    if (p != q) {
+      for (int r = j0 + lane; r < N; r += 64) {
+        const double t0 = pnl[r * NB + q];
+        pnl[r * NB + q] = pnl[r * NB + p];
+        pnl[r * NB + p] = t0;
+      }
+      for (int r = lane; r < j0; r += 64) {
+        const double t0 = A[(size_t)r * N + j0 + q];
+        A[(size_t)r * N + j0 + q] = A[(size_t)r * N + j0 + p];
+        A[(size_t)r * N + j0 + p] = t0;
+      }
+      if (lane == 0) {
+        const int tp = perm[j0 + q];
+        perm[j0 + q] = perm[j0 + p];
+        perm[j0 + p] = tp;
+      }
+    }
+    qr_wave_sync();
+    const double alpha = pnl[j * NB + q];
+    const double nrmv = sqrt(best > 0.0 ? best : 0.0);
+    double beta, tau, scal;
+    if (nrmv == 0.0) {
+      beta = 0.0; tau = 0.0; scal = 0.0;
+      zero_pivot = true;
+    } else {
+      beta = alpha >= 0.0 ? -nrmv : nrmv;
+      tau = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    for (int r = j + 1 + lane; r < N; r += 64) pnl[r * NB + q] *= scal;
+    qr_wave_sync();
+    double d[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) d[c] = 0.0;
+    for (int r = j + lane; r < N; r += 64) {
+      const double v = r == j ? 1.0 : pnl[r * NB + q];
+      const double* row = pnl + r * NB;
+#pragma unroll
+      for (int c = 0; c < NB; ++c) d[c] += v * row[c];
+    }
+#pragma unroll
+    for (int c = 0; c < NB; ++c) d[c] = qr_wave_sum(d[c]) * tau;
+    for (int r = j + lane; r < N; r += 64) {
+      const double v = r == j ? 1.0 : pnl[r * NB + q];
+      double* row = pnl + r * NB;
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (c > q && c < w) row[c] -= v * d[c];
+    }
+    qr_wave_sync();
+    if (lane == 0) {
+      pnl[j * NB + q] = beta;
+      taus[j] = tau;
+    }
+    qr_wave_sync();
   }
 }
 
@@ -163,77 +264,11 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
       pnl[r * nb + q] = q < w ? A[(size_t)r * N + j0 + q] : 0.0;
     }
     __syncthreads();
-    for (int q = 0; q < w; ++q) {
-      const int j = j0 + q;
-      // panel-local pivoting: the remaining panel column of largest updated norm
-      {
-        const int lane = tid & 63, wave = tid >> 6;
-        for (int q2 = q + wave; q2 < w; q2 += QR_WAVES) {
-          double s2 = 0.0;
-          for (int r = j + lane; r < N; r += 64) {
-            const double x = pnl[r * nb + q2];
-            s2 += x * x;
-          }
-          s2 = qr_wave_sum(s2);
-          if (lane == 0) Wt[q2] = s2;
-        }
-        __syncthreads();
-        int p = q;
-        double best = Wt[q];
-        for (int q2 = q + 1; q2 < w; ++q2)
-          if (Wt[q2] > best) { best = Wt[q2]; p = q2; }
-        if (p != q) {
-          for (int r = j0 + tid; r < N; r += QR_THREADS) {
-            const double t0 = pnl[r * nb + q];
-            pnl[r * nb + q] = pnl[r * nb + p];
-            pnl[r * nb + p] = t0;
-          }
-          // R entries of rows above the panel live in A
-          for (int r = tid; r < j0; r += QR_THREADS) {
-            const double t0 = A[(size_t)r * N + j0 + q];
-            A[(size_t)r * N + j0 + q] = A[(size_t)r * N + j0 + p];
-            A[(size_t)r * N + j0 + p] = t0;
-          }
-          if (tid == 0) {
-            const int tp = perm[j0 + q];
-            perm[j0 + q] = perm[j0 + p];
-            perm[j0 + p] = tp;
-          }
-        }
-        __syncthreads();
-      }
-      double s = 0.0;
-      for (int r = j + tid; r < N; r += QR_THREADS) {
-        const double x = pnl[r * nb + q];
-        s += x * x;
-      }
-      const double nrm2 = qr_block_sum(s, red);
-      const double alpha = pnl[j * nb + q];
-      const double nrm = sqrt(nrm2);
-      double beta, tau, scal;
-      if (nrm == 0.0) {
-        beta = 0.0; tau = 0.0; scal = 0.0;
-        zero_pivot = true;
-      } else {
-        beta = alpha >= 0.0 ? -nrm : nrm;
-        tau = (beta - alpha) / beta;
-        scal = 1.0 / (alpha - beta);
-      }
-      for (int r = j + 1 + tid; r < N; r += QR_THREADS) pnl[r * nb + q] *= scal;
-      __syncthreads();
-      if (tid == 0) {
-        pnl[j * nb + q] = beta;
-        taus[j] = tau;
-      }
-      const int lane = tid & 63, wave = tid >> 6;
-      for (int q2 = q + 1 + wave; q2 < w; q2 += QR_WAVES) {
-        double d = 0.0;
-        for (int r = j + lane; r < N; r += 64) d += (r == j ? 1.0 : pnl[r * nb + q]) * pnl[r * nb + q2];
-        d = qr_wave_sum(d) * tau;
-        for (int r = j + lane; r < N; r += 64) pnl[r * nb + q2] -= (r == j ? 1.0 : pnl[r * nb + q]) * d;
-      }
-      __syncthreads();
+    if ((tid >> 6) == 0) {
+      if (nb == 16) qr_panel_wave<16>(pnl, A, N, j0, w, perm, taus, zero_pivot);
+      else qr_panel_wave<8>(pnl, A, N, j0, w, perm, taus, zero_pivot);
     }
+    __syncthreads();
     // R (upper part) and V (below the diagonal) back to A; pnl becomes the explicit V
     for (int e = tid; e < (N - j0) * w; e += QR_THREADS) {
       const int r = j0 + e / w, q = e % w;
@@ -248,7 +283,7 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
     __syncthreads();
     if (j0 + w < N) {
       qr_build_t(pnl, nb, j0, N, w, taus + j0, Tm, Wt);
-      qr_apply_block(A, N, j0, j0 + w, pnl, nb, w, Tm, true, Wt);
+      qr_apply_block(A, N, j0, j0 + w, pnl, nb, w, Tm, true);
     }
   }
 
@@ -277,7 +312,7 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
     }
     __syncthreads();
     qr_build_t(pnl, nb, j0, N, w, taus + j0, Tm, Wt);
-    qr_apply_block(Q, N, j0, j0, pnl, nb, w, Tm, false, Wt);
+    qr_apply_block(Q, N, j0, j0, pnl, nb, w, Tm, false);
   }
   if (tid == 0 && st) {
     if (zero_pivot) atomicOr(&st[tk.chain].flags, 2u);

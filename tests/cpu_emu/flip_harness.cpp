@@ -61,8 +61,12 @@ int main(int argc, char** argv) {
   std::printf("N=%d L=%d K=%d threads=%d fp32=%d chains=%d t=%d lds=%zu\n", N, L, K, threads, fp32, nch, t, lds);
   std::fflush(stdout);
   if (K > 1) {
-    if (fp32) emu_launch(nch, threads, lds, [&] { k_flip_delayed<float>(a); });
-    else emu_launch(nch, threads, lds, [&] { k_flip_delayed<double>(a); });
+#define RUNK(KK)                                                                          \
+  case KK:                                                                                \
+    if (fp32) emu_launch(nch, threads, lds, [&] { k_flip_delayed<float, KK>(a); });       \
+    else emu_launch(nch, threads, lds, [&] { k_flip_delayed<double, KK>(a); });          \
+    break;
+    switch (K) { RUNK(4) RUNK(8) RUNK(16) RUNK(32) default: std::printf("unsupported K\n"); return 2; }
   } else {
     if (fp32) emu_launch(nch, threads, lds, [&] { k_flip_rank1<float>(a); });
     else emu_launch(nch, threads, lds, [&] { k_flip_rank1<double>(a); });

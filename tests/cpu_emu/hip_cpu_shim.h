@@ -88,6 +88,18 @@ inline double __shfl_xor(double v, int mask, int width = 64) {
   return r;
 }
 
+// Lane broadcast/gather within a wavefront: returns var of lane src.
+inline double __shfl(double v, int src, int width = 64) {
+  (void)width;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  EmuWave& s = emu_waves[w];
+  s.a[lane] = v;
+  s.bar->arrive_and_wait();
+  const double r = s.a[src & 63];
+  s.bar->arrive_and_wait();
+  return r;
+}
+
 // Run fn() as a block of `threads` threads for every block index in [0, blocks).
 template <typename F>
 inline void emu_launch(int blocks, int threads, size_t lds_bytes, F fn) {

@@ -171,5 +171,9 @@ def test_engine_refuses_oversized_lds():
     _say("[lds limit]")
     from latticeqmc_amd._native import LqmcError
     k, f = square_case(16, 8, 1.0, 4.0, 0)
-    with pytest.raises(LqmcError, match="LDS"):
+    with pytest.raises(LqmcError, match="delay"):
         SweepEngine([spec_from(k, 1.0, 8, 4.0, f)], mode="dqmc", delay=64, n_stab=4, rng="philox")
+    # 28x28 sites: the QR column panel (8 columns of 784 rows plus workspace) exceeds 64 KB.
+    k, f = square_case(28, 4, 1.0, 4.0, 0)
+    with pytest.raises(LqmcError, match="LDS"):
+        SweepEngine([spec_from(k, 1.0, 4, 4.0, f)], mode="dqmc", delay=16, n_stab=4, rng="philox")
