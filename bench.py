@@ -3,13 +3,13 @@
 Default workload (the configuration the metric is quoted on): 64 independent
 chains of a 16x16 periodic square lattice (N=256), t=1, U=4, mu=0, beta=8,
 dtau=0.1 (L=80), i.i.d. +-1 fields from seeds 0..63, FP64 QR/UDV recompute every
-n_s slices.  One step = one sweep of every chain = 64*256*80 = 1,310,720
+n_s slices (autotuned, default 8).  One step = one sweep of every chain = 64*256*80 = 1,310,720
 attempted single-spin flips.
 
 Forms timed:
   chosen  batched chains, submatrix-delayed rank-k updates (k=16) with the
           rank-k products on the FP64 matrix cores, FP64 wraps, FP64 QR/UDV
-          recompute every 10 slices, whole sweep replayed from a hipGraph
+          recompute every n_s slices, whole sweep replayed from a hipGraph
   plain   the same chains batched, FP64, rank-1 Sherman-Morrison per accepted
           flip, no graph
   fp32    the chosen form with FP32 G, wraps and updates, timed on one sweep
