@@ -71,16 +71,17 @@ struct GemmTask {
 
 // Pre-pivoted blocked Householder QR: W = Q diag(D) V'' with V'' written over W.
 struct QrTask {
-  double* W;   // in: matrix; out: V'' = D^-1 R P^T
+  double* W;   // in: matrix; out: V'' = D^-1 R P^T, or R' = D^-1 R when perm is requested
   double* Q;   // out: orthogonal factor
   double* D;   // out: |diag R|
   double* A;   // scratch N x N (holds R and the reflectors)
-  int* perm;   // optional out: pivot order
+  int* perm;   // optional out: pivot order (W then holds R' unpermuted)
+  double* Tt;  // scratch: compact-WY T of every panel, 16 * (N + 16) doubles
   int N;
   int chain;
 };
 
-// Solve V'' Y = Z for the V'' and pivot order of a QrTask (Z destroyed).
+// Solve R' Y' = Z, Y[perm[r]] = Y'[r], for the R' and pivot order of a QrTask (Z destroyed).
 struct TrsmTask {
   const double* V;
   const int* perm;

@@ -91,7 +91,7 @@ struct Chain {
   double *Wa[2] = {}, *Wb[2] = {};
   double *LU[2] = {}, *LD[2] = {}, *LV[2] = {};
   double *RU[2] = {}, *RD[2] = {}, *RV[2][2] = {};
-  double *Tq[2] = {}, *aug[2] = {};
+  double *Tq[2] = {}, *Tt[2] = {}, *aug[2] = {};
   double *Dx[2] = {}, *Zb[2] = {};
   int* px[2] = {};
   double *eye = nullptr, *ones = nullptr;
@@ -198,6 +198,7 @@ static void layout_chain(const lqmc_ctx* x, Chain& c, char* base, size_t& off) {
       carve(base, off, c.RV[s][0], n2);
       carve(base, off, c.RV[s][1], n2);
       carve(base, off, c.Tq[s], n2);
+      carve(base, off, c.Tt[s], 16 * (N + 16));
       carve(base, off, c.Dx[s], N);
       carve(base, off, c.Zb[s], n2);
       carve(base, off, c.px[s], N);
@@ -400,7 +401,7 @@ static void build_program(lqmc_ctx* x) {
           const bool bottom = (m == c.nb - 1);
           for (int s = 0; s < 2; ++s) {
             const size_t n2 = (size_t)c.N * c.N;
-            q.push_back(QrTask{wcur[ci * 2 + s], c.LU[s] + m * n2, c.LD[s] + (size_t)m * c.N, c.Tq[s], nullptr,
+            q.push_back(QrTask{wcur[ci * 2 + s], c.LU[s] + m * n2, c.LD[s] + (size_t)m * c.N, c.Tq[s], nullptr, c.Tt[s],
                                c.N, ci});
             const double* vin = bottom ? c.eye : c.LV[s] + (m + 1) * n2;
             g2.push_back(mk(wcur[ci * 2 + s], c.N, 0, vin, c.N, 0, c.LV[s] + m * n2, c.N, c.N));
@@ -432,7 +433,7 @@ static void build_program(lqmc_ctx* x) {
           GemmTask bb = mk(VL, c.N, 0, VR, c.N, 1, c.aug[s], c.N, c.N);
           bb.rs = DL; bb.rs_kind = SC_SMALL; bb.cs = DR; bb.cs_kind = SC_SMALL; bb.beta = 1.0;
           g2.push_back(bb);
-          q.push_back(QrTask{c.aug[s], c.aug[s] + n2, c.Dx[s], c.Tq[s], c.px[s], c.N, which[k]});
+          q.push_back(QrTask{c.aug[s], c.aug[s] + n2, c.Dx[s], c.Tq[s], c.px[s], c.Tt[s], c.N, which[k]});
         }
       }
       b.gemm(ops, g1, 0);
@@ -497,9 +498,9 @@ static void build_program(lqmc_ctx* x) {
           g2.push_back(mk(c.Tw[s], c.N, 0, ek, c.N, 0, Gp(c, s), c.N, c.N));
         }
       }
-      b.gemm(x->sweep_ops, g1, fp32);
-      b.gemm(x->sweep_ops, g2, fp32);
-      // right product W = E_K V(l) In
+      // right product W = E_K V(l) In; FP64 and independent of the wrap, so in FP64
+      // sweeps its tasks join the first wrap launch
+      if (fp32) b.gemm(x->sweep_ops, g1, fp32);
       std::vector<int> bnd, ml, rvsel;
       for (int ci = 0; ci < nc; ++ci) {
         Chain& c = x->ch[ci];
@@ -520,12 +521,13 @@ static void build_program(lqmc_ctx* x) {
         if ((t + 1) % ns == 0 || t == c.L - 1) bnd.push_back(ci);
       }
       b.gemm(x->sweep_ops, g1, 0);
+      b.gemm(x->sweep_ops, g2, fp32);
       if (bnd.empty()) continue;
       for (int ci : bnd) {
         Chain& c = x->ch[ci];
         const int nxt = rvcur[ci] < 0 ? 0 : 1 - rvcur[ci];
         for (int s = 0; s < 2; ++s) {
-          q.push_back(QrTask{wcur[ci * 2 + s], c.RU[s], c.RD[s], c.Tq[s], nullptr, c.N, ci});
+          q.push_back(QrTask{wcur[ci * 2 + s], c.RU[s], c.RD[s], c.Tq[s], nullptr, c.Tt[s], c.N, ci});
           const double* vin = rvcur[ci] < 0 ? c.eye : c.RV[s][rvcur[ci]];
           g1.push_back(mk(wcur[ci * 2 + s], c.N, 0, vin, c.N, 0, c.RV[s][nxt], c.N, c.N));
         }

@@ -1,22 +1,27 @@
-// Blocked FP64 factorisations for the UDV stabilisation (one workgroup per matrix).
+// Blocked FP64 factorisations for the UDV stabilisation.
 //
-// k_qr_blocked: pre-pivoted Householder QR, W P = Q R with the columns of W
-//   sorted once by decreasing norm (pre-pivoting, Bai, Lee, Li, Xu 2011), then an
-//   unpivoted blocked factorisation: each panel of nb columns is factored in LDS
-//   and applied to the trailing matrix in compact-WY form
-//   (I - V T V^T)^T A = A - V T^T (V^T A), two passes over the trailing matrix
-//   per panel instead of per column.  Outputs Q (explicit, backward blocked
-//   accumulation), D = |diag R| and V'' = D^-1 R P^T written over W, so that
-//   W = Q diag(D) V''.  The factor lives in the scratch matrix A.
-// k_trsm_perm: solves V'' Y = Z for the V'' of k_qr_blocked (R' = D^-1 R is
-//   upper triangular with unit-modulus diagonal, V''[i][perm[c]] = R'[i][c]),
-//   block back substitution on 16-row blocks.
+// k_qr_blocked (one workgroup per matrix): pre-pivoted Householder QR, W P = Q R
+//   with the columns of W sorted once by decreasing norm (pre-pivoting, Bai, Lee,
+//   Li, Xu 2011) and panel-local column pivoting: each panel of nb columns is
+//   factored in LDS by one wavefront and applied to the trailing matrix in
+//   compact-WY form (I - V T V^T)^T A = A - V T^T (V^T A) on the matrix cores.
+//   Outputs D = |diag R| and V'' = D^-1 R P^T written over W (or R' = D^-1 R and
+//   the pivot order when perm is requested), the reflectors below the diagonal of
+//   the scratch matrix A and the T factor of every panel in Tt.
+// k_qr_formq (one workgroup per 64-column block of Q): Q[:, c0:c1] = H_0 ... H_{c1-1}
+//   E[:, c0:c1] by backward blocked accumulation from A and Tt; the column blocks of
+//   Q are independent, so the explicit Q is formed by many workgroups per matrix.
+// k_trsm_perm (one workgroup per 64-column block): solves R' Y' = Z by block back
+//   substitution on 16-row blocks (off-diagonal updates on the matrix cores) and
+//   scatters Y[perm[r]] = Y'[r].
 #include "common.h"
 
 namespace lqmc {
 
 constexpr int QR_THREADS = 1024;
 constexpr int QR_WAVES = QR_THREADS / 64;
+constexpr int QF_COLS = 64;                 // columns per workgroup in k_qr_formq / k_trsm_perm
+constexpr int QF_THREADS = 64 * QF_COLS / 16;
 constexpr int TRSM_B = 16;
 
 __device__ inline double qr_wave_sum(double v) {
@@ -24,44 +29,7 @@ __device__ inline double qr_wave_sum(double v) {
   return v;
 }
 
-__device__ inline double qr_block_sum(double v, double* red) {
-  v = qr_wave_sum(v);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double r = 0.0;
-#pragma unroll
-  for (int q = 0; q < QR_WAVES; ++q) r += red[q];
-  __syncthreads();
-  return r;
-}
-
 __device__ inline int qr_panel_width(int N) { return N > 256 ? 8 : 16; }
-
-// T (w x w, ld nb) of the compact-WY form of the panel's reflectors (V explicit in pnl).
-__device__ void qr_build_t(const double* pnl, int nb, int r0, int N, int w, const double* tau, double* Tm,
-                           double* S) {
-  for (int e = threadIdx.x; e < w * w; e += QR_THREADS) {
-    const int p = e / w, q = e % w;
-    if (p < q) {
-      double s = 0.0;
-      for (int r = r0; r < N; ++r) s += pnl[r * nb + p] * pnl[r * nb + q];
-      S[p * 64 + q] = s;
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < w; ++i) {
-      Tm[i * nb + i] = tau[i];
-      for (int p = 0; p < i; ++p) {
-        double acc = 0.0;
-        for (int p2 = p; p2 < i; ++p2) acc += Tm[p * nb + p2] * S[p2 * 64 + i];
-        Tm[p * nb + i] = -tau[i] * acc;
-      }
-      for (int p = i + 1; p < w; ++p) Tm[p * nb + i] = 0.0;
-    }
-  }
-  __syncthreads();
-}
 
 typedef double qr_v4d __attribute__((ext_vector_type(4)));
 
@@ -69,24 +37,27 @@ __device__ inline qr_v4d qr_mma(double a, double b, qr_v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// X[r0:N, c_begin:N] <- (I - V T' V^T) X[r0:N, c_begin:N] on the FP64 matrix cores,
-// V = pnl (explicit reflectors, rows r0..N-1, w <= 16 columns), T' = T^T (trans) or T.
-// One wavefront per 16-column tile:  W = V^T X (K = rows), W' = T' W, X -= V W'.
-// In the 16x16x4 FP64 layout the accumulator element v of lane l is row l/16 + 4v,
-// column l%16 -- exactly the B operand lane l supplies for k-step v -- so W and W'
-// chain through registers.  Ends with a workgroup barrier.
-__device__ void qr_apply_block(double* X, int N, int r0, int c_begin, const double* pnl, int nb, int w,
-                               const double* Tm, bool trans) {
+// X[r0:N, c_begin:c_end] <- (I - V T' V^T) X[r0:N, c_begin:c_end] on the FP64 matrix
+// cores, V = pnl (explicit reflectors, rows r0..N-1, w <= 16 columns), T' = T^T (trans)
+// or T.  One wavefront per 16-column tile (NW wavefronts in the workgroup):
+// W = V^T X (K = rows), W' = T' W, X -= V W'.  In the 16x16x4 FP64 layout the
+// accumulator element v of lane l is row l/16 + 4v, column l%16 -- exactly the B operand
+// lane l supplies for k-step v -- so W and W' chain through registers.  Ends with a
+// workgroup barrier.
+template <int NW>
+__device__ void qr_apply_block(double* X, int N, int r0, int c_begin, int c_end, const double* pnl, int nb,
+                               int w, const double* Tm, bool trans) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  const int nct = (N - c_begin + 15) >> 4;
-  for (int ct = wave; ct < nct; ct += QR_WAVES) {
+  const int nct = (c_end - c_begin + 15) >> 4;
+  for (int ct = wave; ct < nct; ct += NW) {
     const int c = c_begin + ct * 16 + li;
+    const bool cok = c < c_end;
     qr_v4d wt = {0.0, 0.0, 0.0, 0.0};
     for (int kk = r0; kk < N; kk += 4) {
       const int r = kk + lk;
       const double a = (r < N && li < w) ? pnl[r * nb + li] : 0.0;
-      const double b = (r < N && c < N) ? X[(size_t)r * N + c] : 0.0;
+      const double b = (r < N && cok) ? X[(size_t)r * N + c] : 0.0;
       wt = qr_mma(a, b, wt);
     }
     qr_v4d w2 = {0.0, 0.0, 0.0, 0.0};
@@ -101,7 +72,7 @@ __device__ void qr_apply_block(double* X, int N, int r0, int c_begin, const doub
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int row = rt + lk + 4 * v;
-        acc[v] = (row < N && c < N) ? X[(size_t)row * N + c] : 0.0;
+        acc[v] = (row < N && cok) ? X[(size_t)row * N + c] : 0.0;
       }
       const int ra = rt + li;
 #pragma unroll
@@ -113,7 +84,7 @@ __device__ void qr_apply_block(double* X, int N, int r0, int c_begin, const doub
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int row = rt + lk + 4 * v;
-        if (row < N && c < N) X[(size_t)row * N + c] = acc[v];
+        if (row < N && cok) X[(size_t)row * N + c] = acc[v];
       }
     }
   }
@@ -212,33 +183,84 @@ __device__ void qr_panel_wave(double* pnl, double* A, int N, int j0, int w, int*
   }
 }
 
+// T (w x w, leading dimension nb) of the compact-WY form I - V T V^T = H_0 ... H_{w-1}
+// of a panel's reflectors (V explicit in pnl, rows r0..N-1).  Wavefront 0 forms
+// S = V^T V on the matrix cores (A and B operands coincide), then lane p computes row p
+// of T in registers from the recurrence T[p][p] = tau_p,
+// T[p][i] = -tau_i * sum_{p <= q < i} T[p][q] S[q][i]  (rows are independent).
+// S is a 16 x 16 LDS scratch.  Ends with a workgroup barrier.
+__device__ void qr_build_t(const double* pnl, int nb, int r0, int N, int w, const double* tau, double* Tm,
+                           double* S) {
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x, li = lane & 15, lk = lane >> 4;
+    qr_v4d acc = {0.0, 0.0, 0.0, 0.0};
+    for (int kk = r0; kk < N; kk += 4) {
+      const int r = kk + lk;
+      const double v = (r < N && li < w) ? pnl[r * nb + li] : 0.0;
+      acc = qr_mma(v, v, acc);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) S[(lk + 4 * v) * 16 + li] = acc[v];
+    qr_wave_sync();
+    if (lane < w) {
+      const int p = lane;
+      double t[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < i; ++q)
+          if (q >= p) s += t[q] * S[q * 16 + i];
+        t[i] = (i < p || i >= w) ? 0.0 : (i == p ? tau[p] : -tau[i] * s);
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+        if (i < w) Tm[p * nb + i] = t[i];
+    }
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restrict__ tasks, ChainState* st) {
   const QrTask tk = tasks[blockIdx.x];
   const int N = tk.N;
   const int nb = qr_panel_width(N);
   double* __restrict__ W = tk.W;
   double* __restrict__ A = tk.A;
-  double* __restrict__ Q = tk.Q;
   LQMC_DYNAMIC_LDS(double, sm);
   double* pnl = sm;                    // [N][nb]
   double* Tm = pnl + (size_t)N * nb;   // [nb][nb]
-  double* Wt = Tm + nb * nb;           // [nb][64] (also S)
-  double* red = Wt + nb * 64;          // [64]
-  double* norms = red + 64;            // [N]
+  double* Ssc = Tm + nb * nb;          // [16][16]
+  double* norms = Ssc + 256;           // [N]
   double* taus = norms + N;            // [N]
   int* perm = reinterpret_cast<int*>(taus + N);  // [N]
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   bool zero_pivot = false, bad = false;
 
-  // pre-pivoting: order columns by decreasing norm (ties by index; NaN last)
-  for (int c = tid; c < N; c += QR_THREADS) {
-    double s = 0.0;
-    for (int r = 0; r < N; ++r) {
-      const double x = W[(size_t)r * N + c];
-      s += x * x;
+  // pre-pivoting: order columns by decreasing norm (ties by index; NaN last).  Lane =
+  // column within a 64-column tile, the wavefronts of a tile split the rows; partial
+  // sums meet in the (not yet used) panel buffer.
+  {
+    const int nct = (N + 63) >> 6;            // <= 16 for N <= 1024
+    const int ng = QR_WAVES / nct;            // row groups per tile
+    const int ct = wave % nct, g = wave / nct;
+    if (g < ng) {
+      const int c = ct * 64 + lane;
+      double s = 0.0;
+      if (c < N)
+        for (int r = g; r < N; r += ng) {
+          const double x = W[(size_t)r * N + c];
+          s += x * x;
+        }
+      if (c < N) pnl[g * N + c] = s;
     }
-    if (!(s >= 0.0)) { s = -1.0; bad = true; }
-    norms[c] = s;
+    __syncthreads();
+    for (int c = tid; c < N; c += QR_THREADS) {
+      double s = 0.0;
+      for (int q = 0; q < ng; ++q) s += pnl[q * N + c];
+      if (!(s >= 0.0)) { s = -1.0; bad = true; }
+      norms[c] = s;
+    }
   }
   __syncthreads();
   for (int c = tid; c < N; c += QR_THREADS) {
@@ -264,7 +286,7 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
       pnl[r * nb + q] = q < w ? A[(size_t)r * N + j0 + q] : 0.0;
     }
     __syncthreads();
-    if ((tid >> 6) == 0) {
+    if (wave == 0) {
       if (nb == 16) qr_panel_wave<16>(pnl, A, N, j0, w, perm, taus, zero_pivot);
       else qr_panel_wave<8>(pnl, A, N, j0, w, perm, taus, zero_pivot);
     }
@@ -281,130 +303,167 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
       else if (rr == q) pnl[(j0 + rr) * nb + q] = 1.0;
     }
     __syncthreads();
-    if (j0 + w < N) {
-      qr_build_t(pnl, nb, j0, N, w, taus + j0, Tm, Wt);
-      qr_apply_block(A, N, j0, j0 + w, pnl, nb, w, Tm, true);
+    qr_build_t(pnl, nb, j0, N, w, taus + j0, Tm, Ssc);
+    double* tout = tk.Tt + (size_t)(j0 / nb) * nb * nb;
+    for (int e = tid; e < nb * nb; e += QR_THREADS) {
+      const int p = e / nb, q = e % nb;
+      tout[e] = (p < w && q < w) ? Tm[e] : 0.0;
     }
+    if (j0 + w < N) qr_apply_block<QR_WAVES>(A, N, j0, j0 + w, N, pnl, nb, w, Tm, true);
   }
 
-  // D = |diag R|, V'' = D^-1 R P^T over W
+  // D = |diag R|; V'' = D^-1 R P^T over W, or R' = D^-1 R (unpermuted) with the pivot order
   for (int i = tid; i < N; i += QR_THREADS) {
     const double d = fabs(A[(size_t)i * N + i]);
-    tk.D[i] = d > 0.0 ? d : 1e-300;
+    norms[i] = d > 0.0 ? d : 1e-300;
+    tk.D[i] = norms[i];
     if (tk.perm) tk.perm[i] = perm[i];
   }
   __syncthreads();
-  for (size_t e = tid; e < (size_t)N * N; e += QR_THREADS) {
-    const int i = (int)(e / N), c = (int)(e % N);
-    W[(size_t)i * N + perm[c]] = c >= i ? A[e] / tk.D[i] : 0.0;
+  if (tk.perm) {
+    for (size_t e = tid; e < (size_t)N * N; e += QR_THREADS) {
+      const int i = (int)(e / N), c = (int)(e % N);
+      W[e] = c >= i ? A[e] / norms[i] : 0.0;
+    }
+  } else {
+    for (size_t e = tid; e < (size_t)N * N; e += QR_THREADS) {
+      const int i = (int)(e / N), c = (int)(e % N);
+      W[(size_t)i * N + perm[c]] = c >= i ? A[e] / norms[i] : 0.0;
+    }
   }
-  // Q = H_0 ... H_{N-1}: backward over panels, Q[j0:, j0:] <- (I - V T V^T) Q[j0:, j0:]
-  for (size_t e = tid; e < (size_t)N * N; e += QR_THREADS) Q[e] = (e / N == e % N) ? 1.0 : 0.0;
-  __syncthreads();
-  for (int j0 = ((N - 1) / nb) * nb; j0 >= 0; j0 -= nb) {
+  if (tid == 0 && st) {
+    if (zero_pivot) atomicOr(&st[tk.chain].flags, 2u);
+    if (bad) atomicOr(&st[tk.chain].flags, 8u);
+  }
+}
+
+// Q[:, c0:c1] of the factorisation left in A / Tt by k_qr_blocked.  Reflector H_j acts on
+// rows >= j only, so the identity columns c0..c1-1 are touched by panels j0 <= c1-1, and
+// panel j0 changes rows and columns >= j0 only.
+__global__ __launch_bounds__(QF_THREADS) void k_qr_formq(const QrTask* __restrict__ tasks, int nblk) {
+  const QrTask tk = tasks[blockIdx.x / nblk];
+  const int N = tk.N;
+  const int c0 = (blockIdx.x % nblk) * QF_COLS;
+  if (c0 >= N) return;
+  const int c1 = min(N, c0 + QF_COLS), cw = c1 - c0;
+  const int nb = qr_panel_width(N);
+  const double* __restrict__ A = tk.A;
+  double* __restrict__ Q = tk.Q;
+  LQMC_DYNAMIC_LDS(double, sm);
+  double* pnl = sm;                    // [N][nb]
+  double* Tm = pnl + (size_t)N * nb;   // [nb][nb]
+  const int tid = threadIdx.x;
+  for (int e = tid; e < N * cw; e += QF_THREADS) {
+    const int r = e / cw, c = c0 + e % cw;
+    Q[(size_t)r * N + c] = r == c ? 1.0 : 0.0;
+  }
+  for (int j0 = ((c1 - 1) / nb) * nb; j0 >= 0; j0 -= nb) {
     const int w = min(nb, N - j0);
-    for (int e = tid; e < (N - j0) * nb; e += QR_THREADS) {
+    for (int e = tid; e < (N - j0) * nb; e += QF_THREADS) {
       const int r = j0 + e / nb, q = e % nb;
       const int rr = r - j0;
       double v = 0.0;
       if (q < w) v = rr < q ? 0.0 : (rr == q ? 1.0 : A[(size_t)r * N + j0 + q]);
       pnl[r * nb + q] = v;
     }
+    const double* tin = tk.Tt + (size_t)(j0 / nb) * nb * nb;
+    for (int e = tid; e < nb * nb; e += QF_THREADS) Tm[e] = tin[e];
     __syncthreads();
-    qr_build_t(pnl, nb, j0, N, w, taus + j0, Tm, Wt);
-    qr_apply_block(Q, N, j0, j0, pnl, nb, w, Tm, false);
+    qr_apply_block<QF_THREADS / 64>(Q, N, j0, max(c0, j0), c1, pnl, nb, w, Tm, false);
   }
-  if (tid == 0 && st) {
-    if (zero_pivot) atomicOr(&st[tk.chain].flags, 2u);
-    if (bad) atomicOr(&st[tk.chain].flags, 8u);
-  }
-  if (bad && st) atomicOr(&st[tk.chain].flags, 8u);
 }
 
-__global__ __launch_bounds__(QR_THREADS) void k_trsm_perm(const TrsmTask* __restrict__ tasks, ChainState* st) {
-  const TrsmTask tk = tasks[blockIdx.x];
+// R' Y' = Z on the columns c0..c1-1 (R' upper triangular, unit-modulus diagonal), then
+// Y[perm[r]][c] = Y'[r][c].  Per 16-row block from the bottom: wavefront t forms
+// Z[blk, tile t] - R'[blk, below] Y'[below, tile t] on the matrix cores, then one lane per
+// column solves the 16 x 16 diagonal block.
+__global__ __launch_bounds__(QF_THREADS) void k_trsm_perm(const TrsmTask* __restrict__ tasks, int nblk,
+                                                          ChainState* st) {
+  const TrsmTask tk = tasks[blockIdx.x / nblk];
   const int N = tk.N;
-  const double* __restrict__ V = tk.V;
-  const int* __restrict__ perm = tk.perm;
+  const int c0 = (blockIdx.x % nblk) * QF_COLS;
+  if (c0 >= N) return;
+  const int c1 = min(N, c0 + QF_COLS);
+  const double* __restrict__ R = tk.V;
   double* __restrict__ Z = tk.Z;
   LQMC_DYNAMIC_LDS(double, sm);
-  double* blk = sm;                                   // [TRSM_B][64]
-  double* rb = blk + TRSM_B * 64;                     // [TRSM_B][TRSM_B] diagonal block of R'
-  int* pm = reinterpret_cast<int*>(rb + TRSM_B * TRSM_B);  // [N]
+  double* blk = sm;                     // [TRSM_B][QF_COLS]
+  double* rb = blk + TRSM_B * QF_COLS;  // [TRSM_B][TRSM_B]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int c = tid; c < N; c += QR_THREADS) pm[c] = perm[c];
-  __syncthreads();
+  const int li = lane & 15, lk = lane >> 4;
+  const int c = c0 + wave * 16 + li;
+  const bool cok = c < c1;
   bool zero = false;
   for (int i0 = ((N - 1) / TRSM_B) * TRSM_B; i0 >= 0; i0 -= TRSM_B) {
     const int b = min(TRSM_B, N - i0);
-    for (int e = tid; e < b * b; e += QR_THREADS) {
-      const int ii = e / b, kk = e % b;
-      rb[ii * TRSM_B + kk] = V[(size_t)(i0 + ii) * N + pm[i0 + kk]];
+    for (int e = tid; e < TRSM_B * TRSM_B; e += QF_THREADS) {
+      const int ii = e / TRSM_B, kk = e % TRSM_B;
+      rb[e] = (ii < b && kk < b) ? R[(size_t)(i0 + ii) * N + i0 + kk] : 0.0;
     }
-    for (int cc = 0; cc < N; cc += 64) {
-      const int c = cc + lane;
-      // rows of this block owned by the wave: ii = wave + 4m
-      double acc[TRSM_B / QR_WAVES];
+    qr_v4d acc;
 #pragma unroll
-      for (int m = 0; m < TRSM_B / QR_WAVES; ++m) {
-        const int ii = wave + QR_WAVES * m;
-        acc[m] = (ii < b && c < N) ? Z[(size_t)(i0 + ii) * N + c] : 0.0;
-      }
-      if (c < N)
-        for (int k = i0 + b; k < N; ++k) {
-          const double zk = Z[(size_t)k * N + c];
-          const int pk = pm[k];
-#pragma unroll
-          for (int m = 0; m < TRSM_B / QR_WAVES; ++m) {
-            const int ii = wave + QR_WAVES * m;
-            if (ii < b) acc[m] -= V[(size_t)(i0 + ii) * N + pk] * zk;
-          }
-        }
-#pragma unroll
-      for (int m = 0; m < TRSM_B / QR_WAVES; ++m) {
-        const int ii = wave + QR_WAVES * m;
-        if (ii < b) blk[ii * 64 + lane] = acc[m];
-      }
-      __syncthreads();
-      if (wave == 0) {
-        for (int ii = b - 1; ii >= 0; --ii) {
-          double s = blk[ii * 64 + lane];
-          for (int kk = ii + 1; kk < b; ++kk) s -= rb[ii * TRSM_B + kk] * blk[kk * 64 + lane];
-          const double d = rb[ii * TRSM_B + ii];
-          if (d == 0.0) zero = true;
-          blk[ii * 64 + lane] = s / d;
-        }
-      }
-      __syncthreads();
-      if (c < N)
-        for (int ii = wave; ii < b; ii += QR_WAVES) Z[(size_t)(i0 + ii) * N + c] = blk[ii * 64 + lane];
-      __syncthreads();
+    for (int v = 0; v < 4; ++v) {
+      const int row = lk + 4 * v;
+      acc[v] = (row < b && cok) ? Z[(size_t)(i0 + row) * N + c] : 0.0;
     }
+    const int ra = i0 + li;
+    for (int k = i0 + b; k < N; k += 4) {
+      const int kr = k + lk;
+      const double a = (li < b && kr < N) ? -R[(size_t)ra * N + kr] : 0.0;
+      const double y = (kr < N && cok) ? Z[(size_t)kr * N + c] : 0.0;
+      acc = qr_mma(a, y, acc);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) blk[(lk + 4 * v) * QF_COLS + wave * 16 + li] = acc[v];
+    __syncthreads();
+    if (tid < QF_COLS) {
+      for (int ii = b - 1; ii >= 0; --ii) {
+        double s = blk[ii * QF_COLS + tid];
+        for (int kk = ii + 1; kk < b; ++kk) s -= rb[ii * TRSM_B + kk] * blk[kk * QF_COLS + tid];
+        const double d = rb[ii * TRSM_B + ii];
+        if (d == 0.0) zero = true;
+        blk[ii * QF_COLS + tid] = s / d;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = lk + 4 * v;
+      if (row < b && cok) Z[(size_t)(i0 + row) * N + c] = blk[row * QF_COLS + wave * 16 + li];
+    }
+    __syncthreads();
   }
-  for (size_t e = tid; e < (size_t)N * N; e += QR_THREADS) {
-    const int r = (int)(e / N), c = (int)(e % N);
-    tk.Y[(size_t)pm[r] * N + c] = Z[e];
+  const int* __restrict__ perm = tk.perm;
+  for (int e = tid; e < N * QF_COLS; e += QF_THREADS) {
+    const int r = e / QF_COLS, cc = c0 + e % QF_COLS;
+    if (cc < c1) tk.Y[(size_t)perm[r] * N + cc] = Z[(size_t)r * N + cc];
   }
   if (zero && st) atomicOr(&st[tk.chain].flags, 2u);
 }
 
 size_t qr_blocked_shmem(int nmax) {
   const size_t nb = nmax > 256 ? 8 : 16;
-  return sizeof(double) * (nmax * nb + nb * nb + nb * 64 + 64 + 2 * (size_t)nmax) + sizeof(int) * nmax;
+  return sizeof(double) * (nmax * nb + nb * nb + 256 + 2 * (size_t)nmax) + sizeof(int) * nmax;
 }
 
-size_t trsm_shmem(int nmax) {
-  return sizeof(double) * (TRSM_B * 64 + TRSM_B * TRSM_B) + sizeof(int) * nmax;
+size_t qr_formq_shmem(int nmax) {
+  const size_t nb = nmax > 256 ? 8 : 16;
+  return sizeof(double) * (nmax * nb + nb * nb);
 }
+
+size_t trsm_shmem(int) { return sizeof(double) * (TRSM_B * QF_COLS + TRSM_B * TRSM_B); }
 
 void launch_qr_blocked(const QrTask* d, int n, int nmax, ChainState* st, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_qr_blocked, dim3(n), dim3(QR_THREADS), qr_blocked_shmem(nmax), s, d, st);
+  const int nblk = (nmax + QF_COLS - 1) / QF_COLS;
+  hipLaunchKernelGGL(k_qr_formq, dim3(n * nblk), dim3(QF_THREADS), qr_formq_shmem(nmax), s, d, nblk);
 }
 
 void launch_trsm(const TrsmTask* d, int n, int nmax, ChainState* st, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_trsm_perm, dim3(n), dim3(QR_THREADS), trsm_shmem(nmax), s, d, st);
+  const int nblk = (nmax + QF_COLS - 1) / QF_COLS;
+  hipLaunchKernelGGL(k_trsm_perm, dim3(n * nblk), dim3(QF_THREADS), trsm_shmem(nmax), s, d, nblk, st);
 }
 
 }  // namespace lqmc

@@ -1,6 +1,7 @@
 // Numerical and bounds check of the blocked QR and triangular-solve kernels in a
 // CPU build: graded test matrices W = M diag(d) (d over many decades, like the
-// UDV chain), checks W = Q diag(D) V'', Q^T Q = I and V'' Y = Z.
+// UDV chain), checks W = Q diag(D) V'' (no pivot request), W P = Q diag(D) R' (pivot
+// order requested), Q^T Q = I and the permuted triangular solve.
 // usage: qr_harness N decades seed
 #include "hip_cpu_shim.h"
 #include "../../latticeqmc_amd/csrc/qr.hip"
@@ -28,18 +29,40 @@ int main(int argc, char** argv) {
   double* Q = static_cast<double*>(std::malloc(n2 * 8));
   double* D = static_cast<double*>(std::malloc(N * 8));
   double* A = static_cast<double*>(std::malloc(n2 * 8));
+  double* Tt = static_cast<double*>(std::malloc(16 * (N + 16) * 8));
   int* perm = static_cast<int*>(std::malloc(N * sizeof(int)));
   ChainState st{};
-  QrTask qt{W, Q, D, A, perm, N, 0};
+  const int nblk = (N + QF_COLS - 1) / QF_COLS;
+  // pass 1 without a pivot request: W <- V'' = D^-1 R P^T, W0 = Q diag(D) V''
+  std::vector<double> Wv(W0);
+  QrTask qv{Wv.data(), Q, D, A, nullptr, Tt, N, 0};
+  emu_launch(1, QR_THREADS, qr_blocked_shmem(N), [&] { k_qr_blocked(&qv, &st); });
+  emu_launch(nblk, QF_THREADS, qr_formq_shmem(N), [&] { k_qr_formq(&qv, nblk); });
+  double recv = 0.0;
+  for (int c = 0; c < N; ++c) {
+    double cn = 0.0, en = 0.0;
+    for (int r = 0; r < N; ++r) {
+      double s = 0.0;
+      for (int k = 0; k < N; ++k) s += Q[(size_t)r * N + k] * D[k] * Wv[(size_t)k * N + c];
+      const double w0 = W0[(size_t)r * N + c];
+      cn += w0 * w0;
+      en += (s - w0) * (s - w0);
+    }
+    recv = std::max(recv, std::sqrt(en / cn));
+  }
+  // pass 2 with the pivot order: W <- R' = D^-1 R, W0[:, perm[c]] = Q diag(D) R'[:, c]
+  std::fill(Q, Q + n2, 0.0);
+  QrTask qt{W, Q, D, A, perm, Tt, N, 0};
   emu_launch(1, QR_THREADS, qr_blocked_shmem(N), [&] { k_qr_blocked(&qt, &st); });
-  // reconstruction, column-relative: |W - Q D V''|_col / |W|_col
+  emu_launch(nblk, QF_THREADS, qr_formq_shmem(N), [&] { k_qr_formq(&qt, nblk); });
+  // reconstruction, column-relative: |W - Q D R'|_col / |W|_col
   double rec = 0.0, orth = 0.0;
   for (int c = 0; c < N; ++c) {
     double cn = 0.0, en = 0.0;
     for (int r = 0; r < N; ++r) {
       double s = 0.0;
       for (int k = 0; k < N; ++k) s += Q[(size_t)r * N + k] * D[k] * W[(size_t)k * N + c];
-      const double w0 = W0[(size_t)r * N + c];
+      const double w0 = W0[(size_t)r * N + perm[c]];
       cn += w0 * w0;
       en += (s - w0) * (s - w0);
     }
@@ -51,27 +74,34 @@ int main(int argc, char** argv) {
       for (int k = 0; k < N; ++k) s += Q[(size_t)k * N + i] * Q[(size_t)k * N + j];
       orth = std::max(orth, std::fabs(s - (i == j ? 1.0 : 0.0)));
     }
-  // V'' must have unit-modulus pivots and bounded entries (graded)
-  double vmax = 0.0;
-  for (size_t e = 0; e < n2; ++e) vmax = std::max(vmax, std::fabs(W[e]));
-  // triangular solve V'' Y = Z
+  // R' must be upper triangular with unit-modulus pivots and bounded entries (graded)
+  double vmax = 0.0, low = 0.0, dia = 0.0;
+  for (int i = 0; i < N; ++i)
+    for (int c = 0; c < N; ++c) {
+      const double v = W[(size_t)i * N + c];
+      vmax = std::max(vmax, std::fabs(v));
+      if (c < i) low = std::max(low, std::fabs(v));
+      if (c == i) dia = std::max(dia, std::fabs(std::fabs(v) - 1.0));
+    }
+  // triangular solve R' Y' = Z, Y[perm[r]] = Y'[r]  <=>  V'' Y = Z with V''[i][perm[c]] = R'[i][c]
   double* Z = static_cast<double*>(std::malloc(n2 * 8));
   double* Y = static_cast<double*>(std::malloc(n2 * 8));
   std::vector<double> Z0(n2);
   for (size_t e = 0; e < n2; ++e) Z0[e] = Z[e] = g(rng);
   TrsmTask tt{W, perm, Z, Y, N, 0};
-  emu_launch(1, QR_THREADS, trsm_shmem(N), [&] { k_trsm_perm(&tt, &st); });
+  emu_launch(nblk, QF_THREADS, trsm_shmem(N), [&] { k_trsm_perm(&tt, nblk, &st); });
   double res = 0.0, zn = 0.0;
   for (int r = 0; r < N; ++r)
     for (int c = 0; c < N; ++c) {
       double s = 0.0;
-      for (int k = 0; k < N; ++k) s += W[(size_t)r * N + k] * Y[(size_t)k * N + c];
+      for (int k = 0; k < N; ++k) s += W[(size_t)r * N + k] * Y[(size_t)perm[k] * N + c];
       res = std::max(res, std::fabs(s - Z0[(size_t)r * N + c]));
       zn = std::max(zn, std::fabs(Z0[(size_t)r * N + c]));
     }
-  std::printf("N=%d decades=%g recon %.2e orth %.2e max|V''| %.3f trsm_res %.2e flags %u\n", N, decades, rec, orth,
-              vmax, res / zn, st.flags);
-  const bool ok = rec < 1e-12 && orth < 1e-12 && vmax < 1e3 && res / zn < 1e-9 && st.flags == 0;
+  std::printf("N=%d decades=%g recon(V'') %.2e recon(R') %.2e orth %.2e max|R'| %.3f lower %.1e |diag|-1 %.1e "
+              "trsm_res %.2e flags %u\n", N, decades, recv, rec, orth, vmax, low, dia, res / zn, st.flags);
+  const bool ok = recv < 1e-12 && rec < 1e-12 && orth < 1e-12 && vmax < 1e3 && low == 0.0 && dia < 1e-12 &&
+                  res / zn < 1e-9 && st.flags == 0;
   std::printf(ok ? "ok\n" : "FAIL\n");
   return ok ? 0 : 1;
 }

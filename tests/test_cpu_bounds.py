@@ -1,4 +1,4 @@
-"""CPU checks of the flip kernels: the HIP sources are compiled for the host with a
+"""CPU checks of the flip and QR kernels: the HIP sources are compiled for the host with a
 small emulation layer (tests/cpu_emu/hip_cpu_shim.h) and run at the sizes the
 engine uses -- under AddressSanitizer with every buffer an exact-size allocation
 (bounds), and under ThreadSanitizer (cross-wavefront races between barriers, such
@@ -64,12 +64,11 @@ def test_flip_kernel_race_free(harness_tsan, args):
     assert "WARNING: ThreadSanitizer" not in r.stderr, r.stderr[:4000]
 
 
-@pytest.fixture(scope="module")
-def qr_harness(tmp_path_factory):
+def _build_qr(tmp_path_factory, sanitizer):
     if not os.path.exists(CLANG):
         pytest.skip("host clang++ with sanitizers not available")
-    out = str(tmp_path_factory.mktemp("emu") / "qr_harness")
-    cmd = [CLANG, "-std=c++20", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer",
+    out = str(tmp_path_factory.mktemp("emu") / f"qr_harness_{sanitizer}")
+    cmd = [CLANG, "-std=c++20", "-O1", "-g", f"-fsanitize={sanitizer}", "-fno-omit-frame-pointer",
            "-I", os.path.join(ROOT, "include"), os.path.join(HERE, "cpu_emu", "qr_harness.cpp"),
            "-o", out, "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -77,10 +76,32 @@ def qr_harness(tmp_path_factory):
     return out
 
 
+@pytest.fixture(scope="module")
+def qr_harness(tmp_path_factory):
+    return _build_qr(tmp_path_factory, "address")
+
+
+@pytest.fixture(scope="module")
+def qr_harness_tsan(tmp_path_factory):
+    return _build_qr(tmp_path_factory, "thread")
+
+
 @pytest.mark.parametrize("args", ["4 2 1", "37 20 3", "100 40 5", "257 40 6"])
 def test_blocked_qr_and_trsm(qr_harness, args):
-    """W = Q diag(D) V'' to 1e-12 column-relative on graded matrices, Q orthogonal,
-    V'' bounded, V'' Y = Z solved to 1e-9, all accesses in bounds."""
+    """QR, the column-block Q formation and the permuted triangular solve on graded
+    matrices: W = Q diag(D) V'' and W P = Q diag(D) R' to 1e-12 column-relative, Q
+    orthogonal, R' upper triangular with unit-modulus diagonal, R' Y' = Z to 1e-9,
+    all accesses in bounds."""
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
     r = subprocess.run([qr_harness, *args.split()], capture_output=True, text=True, env=env, timeout=900)
     assert r.returncode == 0 and "ok" in r.stdout, (r.stdout + r.stderr)[-3000:]
+
+
+@pytest.mark.parametrize("args", ["70 20 2", "257 20 6"])
+def test_blocked_qr_race_free(qr_harness_tsan, args):
+    """No cross-wavefront race in the QR, Q-formation and solve kernels (the 257 case
+    runs the 8-column panels)."""
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=0 report_signal_unsafe=0")
+    r = subprocess.run([qr_harness_tsan, *args.split()], capture_output=True, text=True, env=env, timeout=1500)
+    assert "ok" in r.stdout, (r.stdout + r.stderr)[-3000:]
+    assert "WARNING: ThreadSanitizer" not in r.stderr, r.stderr[:4000]
