@@ -4,8 +4,16 @@
 #include <hip/hip_runtime.h>
 // Dynamic LDS of a kernel (redirected to a host buffer by the CPU bounds-check build).
 #define LQMC_DYNAMIC_LDS(type, name) extern __shared__ type name[]
+// Global address space: loads through such pointers are global_load (vector-memory
+// counter only) rather than flat loads, which also hold the LDS counter.
+#define LQMC_GLOBAL __attribute__((address_space(1)))
 #endif
 #include <stdint.h>
+
+template <typename T>
+__device__ inline LQMC_GLOBAL T* gptr(T* p) {
+  return (LQMC_GLOBAL T*)p;
+}
 
 namespace lqmc {
 

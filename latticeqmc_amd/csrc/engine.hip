@@ -39,6 +39,7 @@ void launch_flip(int fp32, int delayed, const FlipArgs& a, int nchains, int thre
                  hipStream_t s);
 size_t flip_delayed_shmem(int nmax, int k, int threads, int fp32);
 size_t flip_delayed_workspace(int n, int k, int fp32);
+int flip_delayed_threads_limit(int k);
 size_t qr_blocked_shmem(int nmax);
 }  // namespace lqmc
 
@@ -631,9 +632,16 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
   // bounds the supported delay for a given thread count.
   const int maxsh = 65536;
   const bool fp32 = o.precision == LQMC_FP32;
+  const int tmax = o.delay > 1 ? flip_delayed_threads_limit(o.delay) : 1024;
   if (o.flip_threads > 0) x->flip_threads = o.flip_threads;
-  else x->flip_threads = x->nmax >= 128 ? 1024 : 256;
-  if (x->flip_threads % 64 || x->flip_threads > 1024) { delete x; return fail(LQMC_EINVAL, "flip_threads must be a multiple of 64, <= 1024"); }
+  else x->flip_threads = x->nmax >= 128 ? tmax : 256;
+  const int tmin = o.delay > 1 ? 256 : 64;
+  if (x->flip_threads % 64 || x->flip_threads > tmax || x->flip_threads < tmin) {
+    char buf[128];
+    snprintf(buf, sizeof buf, "flip_threads must be a multiple of 64 in [%d, %d] for this delay", tmin, tmax);
+    delete x;
+    return fail(LQMC_EINVAL, buf);
+  }
   if (o.delay > 1) x->flip_shmem = flip_delayed_shmem(x->nmax, o.delay, x->flip_threads, fp32);
   else x->flip_shmem = sizeof(double) * (4 * (size_t)x->nmax + x->flip_threads);
   if ((int)x->flip_shmem > maxsh) {

@@ -22,6 +22,27 @@ namespace lqmc {
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
+// p[ok ? i : 0], then 0 where !ok: the load is unconditional (no branch, so loads of
+// a loop body can all be in flight together) and always addresses valid memory.
+template <typename T>
+__device__ inline T ld0(const T* p, size_t i, bool ok) {
+  const T v = p[ok ? i : 0];
+  return ok ? v : T(0);
+}
+template <typename T>
+__device__ inline T ldg0(const LQMC_GLOBAL T* p, size_t i, bool ok) {
+  const T v = p[ok ? i : 0];
+  return ok ? v : T(0);
+}
+
+// Value of lane l (wavefront-uniform) broadcast through scalar registers.
+__device__ inline double readlane_f64(double v, int l) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // ---- Philox4x32-10 (Salmon et al., SC'11) --------------------------------------------------
 __device__ inline double philox_uniform(uint64_t seed, uint64_t q) {
   uint32_t c0 = (uint32_t)q, c1 = (uint32_t)(q >> 32), c2 = 0u, c3 = 0x51D3u;
@@ -236,36 +257,6 @@ template <> struct MfmaOp<float> {
   __device__ static inline int row(int lane, int r) { return ((lane >> 4) << 2) + r; }
 };
 
-// G (N x N, ld N) += P (Np x K4, ld pld) * Z (K4 x Np, ld Np); Np = N rounded up to 16,
-// K4 = rank rounded up to 4 with zero padding in P and Z.
-template <typename T>
-__device__ void rank_update(T* G, int N, int Np, const T* P, int pld, const T* Z, int K4) {
-  using acc_t = typename MfmaOp<T>::acc_t;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  const int tn = Np >> 4;
-  const int ntile = tn * tn;
-  for (int tile = wave; tile < ntile; tile += nw) {
-    const int r0 = (tile / tn) << 4, c0 = (tile % tn) << 4;
-    const int col = c0 + (lane & 15);
-    acc_t acc;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int row = r0 + MfmaOp<T>::row(lane, q);
-      acc[q] = (row < N && col < N) ? G[(size_t)row * N + col] : T(0);
-    }
-    for (int kk = 0; kk < K4; kk += 4) {
-      const T x = P[(size_t)(r0 + (lane & 15)) * pld + kk + (lane >> 4)];
-      const T y = Z[(size_t)(kk + (lane >> 4)) * Np + col];
-      acc = MfmaOp<T>::mma(x, y, acc);
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int row = r0 + MfmaOp<T>::row(lane, q);
-      if (row < N && col < N) G[(size_t)row * N + col] = acc[q];
-    }
-  }
-}
-
 // Orders LDS traffic between the lanes of one wavefront (the serial phase runs in wave 0).
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -274,69 +265,130 @@ __device__ inline void wave_sync() {
 }
 
 // Per block of k sites (k a multiple of 4):
-//  1. the k x k diagonal blocks of both spins, the block's field entries and its
-//     uniforms go to LDS;
-//  2. wave 0 runs the k attempts on the blocks (rank-1 Sherman-Morrison on k x k),
-//     recording for accepted j the block parts of x_j = c_j G^(j)[:, a_j] and
-//     y_j = e_{a_j} - G^(j)[a_j, :]^T;
+//  1. the block's field entries and uniforms go to LDS;
+//  2. wave 0 runs the k attempts on the k x k diagonal blocks of both spins, held in
+//     its registers (rank-1 Sherman-Morrison on k x k), recording for accepted j the
+//     block parts of x_j = c_j G^(j)[:, a_j] and y_j = e_{a_j} - G^(j)[a_j, :]^T;
 //  3. with U[p][j] = y_p(a_j), L[p][j] = x_p(a_j) (p < j), the full factors are
-//     X = G0[:, A] C (I + U C)^-1 and Y = (E_A - G0[A, :]^T) (I - L)^-1; wave 0
-//     forms the unit-triangular inverses and scatters Mx = C (I + U C)^-1 and
+//     X = G0[:, A] C (I + U C)^-1 and Y = (E_A - G0[A, :]^T) (I - L)^-1; four waves form
+//     the unit-triangular inverses and scatter Mx = C (I + U C)^-1 and
 //     My = (I - L)^-1 onto the block's rows (k x m, zero on rejected sites);
-//  4. P = -G0[:, B] Mx and Z = My^T (E_B - G0[B, :]) on the matrix cores, from
-//     contiguous strips of G0;
-//  5. G <- G + P Z on the matrix cores (= G0 - sum_j x_j y_j^T).
-template <typename T>
-__device__ void strip_products(const T* G, int N, int Np, int b0, int kb, int m, int K4, const double* Mx,
-                               const double* My, int ldm, T* P, int pld, T* Z) {
+//  4. G <- G0 + P Z with P = -G0[:, B] Mx and Z = My^T (E_B - G0[B, :]) on the matrix
+//     cores (= G0 - sum_j x_j y_j^T), block_update below.
+// Rank-m update of both spins' G by an accepted block of sites B = b0 .. b0+kb-1:
+//   G_s += P_s Z_s,  P_s = -G0_s[:, B] Mx_s (N x m),  Z_s = My_s^T (E_B - G0_s[B, :]) (m x N),
+// with Mx_s, My_s the k x m factors scattered on the block rows (LDS, leading dimension K).
+// In the 16x16x4 matrix-core layout, lane l's accumulator element v of a 16 x 16 product
+// is exactly what lane l supplies as the A (or B) operand of k-step v of a following
+// product, with k = l/16 + 4v (FP64) or 4(l/16) + v (FP32) -- any consistent k order sums
+// the same terms.  So
+//  phase 1 (all wavefronts): the Z_s tiles (16 accepts x 16 columns) are formed on the
+//    matrix cores and stored to the workspace in that register order; lane l later reads
+//    back its own element at ((tile * 4 + v) * 64 + l), a coalesced load;
+//  phase 2 (after a workgroup barrier): each wavefront owns whole 16-row strips of G_s,
+//    forms P_s^T for its strip (in operand order, kept in registers) and streams the
+//    strip's 16 x 16 tiles through G += P Z, RU tiles at a time with every load of a group
+//    issued before its first matrix-core op.
+// Phase 1 reads rows B of G0 before any strip is updated; phase 2 reads G0[strip, B] and
+// updates only its own strip.
+template <typename T, int K, int RU>
+__device__ void block_update(T* G, size_t n2, int N, int Np, int b0, int kb, int m, const double* Mx,
+                             const double* My, T* zws) {
   using acc_t = typename MfmaOp<T>::acc_t;
+  constexpr int JT = (K + 15) / 16;  // 16-wide tiles over the accepts
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  const int jt = (m + 15) >> 4;         // 16-wide tiles over the accepted columns
-  const int rt = Np >> 4;
-  // P tiles: (rt x jt);  Z tiles: (jt x rt)
-  for (int tile = wave; tile < 2 * rt * jt; tile += nw) {
-    const bool isP = tile < rt * jt;
-    const int tt = isP ? tile : tile - rt * jt;
-    acc_t acc = acc_t{0, 0, 0, 0};
-    if (isP) {
-      const int r0 = (tt / jt) << 4, j0 = (tt % jt) << 4;
-      const int r = r0 + (lane & 15), j = j0 + (lane & 15);
-      for (int kk = 0; kk < kb; kk += 4) {
-        const int q = kk + (lane >> 4);
-        const T x = (r < N && q < kb) ? G[(size_t)r * N + b0 + q] : T(0);
-        const T y = (q < kb && j < m) ? (T)Mx[q * ldm + j] : T(0);
-        acc = MfmaOp<T>::mma(x, y, acc);
-      }
-      const int col = j0 + (lane & 15);
+  const int li = lane & 15, lk = lane >> 4;
+  const int nct = Np >> 4;
+  const int jtu = (m + 15) >> 4;
+  LQMC_GLOBAL T* Gg = gptr(G);
+  LQMC_GLOBAL T* Zg = gptr(zws);
+  for (int t = wave; t < 2 * nct * JT; t += nw) {
+    const int sp = t / (nct * JT), ct = (t / JT) % nct, jt = t % JT;
+    if (jt >= jtu) continue;
+    const double* Mys = My + sp * K * K;
+    const LQMC_GLOBAL T* Gs = Gg + sp * n2;
+    const int c = (ct << 4) + li, a = (jt << 4) + li;
+    acc_t z = acc_t{0, 0, 0, 0};
+    for (int kk = 0; kk < kb; kk += 4) {
+      const int q = kk + lk;
+      const T x = (T)ld0(Mys, (size_t)q * K + a, q < kb && a < m);
+      T y = -ldg0(Gs, (size_t)(b0 + q) * N + c, q < kb && c < N);
+      if (q < kb && c == b0 + q) y = y + T(1);
+      z = MfmaOp<T>::mma(x, y, z);
+    }
+    LQMC_GLOBAL T* zo = Zg + (size_t)t * 256 + lane;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = r0 + MfmaOp<T>::row(lane, v);
-        if (col < K4) P[(size_t)row * pld + col] = -acc[v];
-      }
-    } else {
-      const int j0 = (tt / rt) << 4, c0 = (tt % rt) << 4;
-      const int j = j0 + (lane & 15), cc = c0 + (lane & 15);
-      for (int kk = 0; kk < kb; kk += 4) {
-        const int q = kk + (lane >> 4);
-        const T x = (q < kb && j < m) ? (T)My[q * ldm + j] : T(0);
-        T y = T(0);
-        if (q < kb && cc < N) {
-          y = -G[(size_t)(b0 + q) * N + cc];
-          if (cc == b0 + q) y = y + T(1);
+    for (int v = 0; v < 4; ++v) zo[v * 64] = z[v];
+  }
+  __syncthreads();
+  for (int rs = wave; rs < 2 * nct; rs += nw) {
+    const int sp = rs / nct, r0 = (rs % nct) << 4;
+    const double* Mxs = Mx + sp * K * K;
+    LQMC_GLOBAL T* Gs = Gg + sp * n2;
+    const int r = r0 + li;
+    acc_t pt[JT];
+#pragma unroll
+    for (int jt = 0; jt < JT; ++jt) {
+      pt[jt] = acc_t{0, 0, 0, 0};
+      if (jt < jtu) {
+        const int a = (jt << 4) + li;
+        for (int kk = 0; kk < kb; kk += 4) {
+          const int q = kk + lk;
+          const T x = (T)ld0(Mxs, (size_t)q * K + a, q < kb && a < m);
+          const T y = ldg0(Gs, (size_t)r * N + b0 + q, r < N && q < kb);
+          pt[jt] = MfmaOp<T>::mma(x, y, pt[jt]);
         }
-        acc = MfmaOp<T>::mma(x, y, acc);
+        pt[jt] = -pt[jt];
+      }
+    }
+    for (int c0 = 0; c0 < nct; c0 += RU) {
+      acc_t acc[RU];
+      T zv[RU][JT][4];
+#pragma unroll
+      for (int u = 0; u < RU; ++u) {
+        const int ct = c0 + u;
+        const int col = (ct << 4) + li;
+        const bool ok = ct < nct && col < N;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = r0 + MfmaOp<T>::row(lane, v);
+          acc[u][v] = ldg0(Gs, (size_t)row * N + col, ok && row < N);
+        }
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            zv[u][jt][v] = ldg0(Zg, ((size_t)((sp * nct + ct) * JT + jt) * 4 + v) * 64 + lane,
+                                ct < nct && jt < jtu);
       }
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = j0 + MfmaOp<T>::row(lane, v);
-        if (row < K4) Z[(size_t)row * Np + cc] = acc[v];
+      for (int u = 0; u < RU; ++u)
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt)
+          if (jt < jtu) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u] = MfmaOp<T>::mma(pt[jt][v], zv[u][jt][v], acc[u]);
+          }
+#pragma unroll
+      for (int u = 0; u < RU; ++u) {
+        const int ct = c0 + u;
+        const int col = (ct << 4) + li;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = r0 + MfmaOp<T>::row(lane, v);
+          if (ct < nct && row < N && col < N) Gs[(size_t)row * N + col] = acc[u][v];
+        }
       }
     }
   }
 }
 
+// Workgroup size bound of the delayed kernel: blocks of 16 or more sites keep their
+// k x k work in registers that only fit the 256-VGPR budget of a 512-thread workgroup.
+constexpr int flip_delayed_max_threads(int k) { return k >= 16 ? 512 : 1024; }
+
 template <typename T, int K>
-__global__ __launch_bounds__(1024) void k_flip_delayed(FlipArgs a) {
+__global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(FlipArgs a) {
   static_assert(K >= 4 && K <= 32 && K % 4 == 0, "block size");
   const int c = blockIdx.x;
   const ChainDev ch = a.chains[c];
@@ -344,13 +396,10 @@ __global__ __launch_bounds__(1024) void k_flip_delayed(FlipArgs a) {
   const int N = ch.N;
   const int l = ch.L - 1 - a.t;
   const int Np = (N + 15) & ~15;
-  constexpr int K16 = (K + 15) & ~15;
-  const int pld = K16 + 1;
   const size_t n2 = (size_t)N * N;
   T* __restrict__ G = static_cast<T*>(ch.G);
   int8_t* fld = ch.field + (size_t)l * N;
-  // workspace per spin: P (Np x pld) then Z (K16 x Np), form precision
-  const size_t spT = (size_t)Np * pld + (size_t)K16 * Np;
+  // workspace: the Z tiles of both spins in register order (2 x K16 x Np, form precision)
   T* wsT = static_cast<T*>(ch.dws);
 
   LQMC_DYNAMIC_LDS(double, sm);
@@ -379,53 +428,66 @@ __global__ __launch_bounds__(1024) void k_flip_delayed(FlipArgs a) {
     }
     __syncthreads();
     if (wave == 0) {
-      // lane (sp, q) = (lane / K, lane % K) holds column q of spin sp's k x k block
-      const int sp = lane / K < 2 ? lane / K : 1;
+      // The k x k blocks of both spins live in wave 0's registers: lane
+      // (sp, h, q) = (lane / (K H), lane / K % H, lane % K) holds rows h R .. h R + R-1
+      // of column q of spin sp (H lanes per column, R = K / H rows each).  The loop over
+      // the block's sites stays rolled; row entries are picked by select chains.
+      constexpr int H = (64 / (2 * K)) < K ? (64 / (2 * K)) : K;
+      constexpr int R = K / H;
+      const int sp = lane / (K * H) < 2 ? lane / (K * H) : 1;
+      const int h = (lane / K) % H;
       const int q = lane % K;
-      const bool live = lane < 2 * K && q < kb;
-      double col[K];
+      const bool live = lane < 2 * K * H && q < kb;
+      double col[R];
 #pragma unroll
-      for (int p = 0; p < K; ++p)
-        col[p] = (live && p < kb) ? (double)G[sp * n2 + (size_t)(b0 + p) * N + b0 + q] : 0.0;
+      for (int i = 0; i < R; ++i) {
+        const int p = h * R + i;
+        col[i] = (live && p < kb) ? (double)G[sp * n2 + (size_t)(b0 + p) * N + b0 + q] : 0.0;
+      }
       int m = 0;
+#pragma unroll 1
+      for (int jj = 0; jj < kb; ++jj) {
+        const int hj = jj / R, ij = jj % R;
+        double cj = 0.0;  // G(current)[jj][q] of my spin, on the lanes with h == hj
 #pragma unroll
-      for (int jj = 0; jj < K; ++jj) {
-        if (jj < kb) {
-          const double gu = __shfl(col[jj], jj, 64);
-          const double gd = __shfl(col[jj], K + jj, 64);
-          const int s = sb[jj];
-          const double ep = s > 0 ? ch.e2p : ch.e2m;
-          const double em = s > 0 ? ch.e2m : ch.e2p;
-          const double d_up = 1.0 + (1.0 - gu) * (ep - 1.0);
-          const double d_dn = 1.0 + (1.0 - gd) * (em - 1.0);
-          const double ratio = d_up * d_dn;
-          const bool acc = ub[jj] <= ratio;
-          if (!isfinite(ratio)) bad = true;
-          if (a.trace && lane == 0) {
-            ch.tr_ratio[(size_t)a.t * N + b0 + jj] = ratio;
-            ch.tr_acc[(size_t)a.t * N + b0 + jj] = acc ? 1 : 0;
-          }
-          if (acc) {
-            const double cu = (ep - 1.0) / d_up, cd = (em - 1.0) / d_dn;
-            const double cfs = sp == 0 ? cu : cd;
-            const int src = sp * K + jj;  // lane holding x = column jj of my spin
-            double yq = -col[jj];
-            if (q == jj) yq = yq + 1.0;
+        for (int i = 0; i < R; ++i)
+          if (i == ij) cj = col[i];
+        const double gu = readlane_f64(cj, hj * K + jj);
+        const double gd = readlane_f64(cj, K * H + hj * K + jj);
+        const int s = sb[jj];
+        const double ep = s > 0 ? ch.e2p : ch.e2m;
+        const double em = s > 0 ? ch.e2m : ch.e2p;
+        const double d_up = 1.0 + (1.0 - gu) * (ep - 1.0);
+        const double d_dn = 1.0 + (1.0 - gd) * (em - 1.0);
+        const double ratio = d_up * d_dn;
+        const bool acc = ub[jj] <= ratio;
+        if (!isfinite(ratio)) bad = true;
+        if (a.trace && lane == 0) {
+          ch.tr_ratio[(size_t)a.t * N + b0 + jj] = ratio;
+          ch.tr_acc[(size_t)a.t * N + b0 + jj] = acc ? 1 : 0;
+        }
+        if (acc) {
+          const double cu = (ep - 1.0) / d_up, cd = (em - 1.0) / d_dn;
+          const double cfs = sp == 0 ? cu : cd;
+          // y = e_jj - G[jj, :]^T: column q's entry sits on lane (sp, hj, q)
+          double yq = -__shfl(cj, sp * K * H + hj * K + q, 64);
+          if (q == jj) yq = yq + 1.0;
+          const int src = sp * K * H + h * K + jj;  // my rows of x = column jj of my spin
 #pragma unroll
-            for (int p = 0; p < K; ++p) {
-              const double xp = __shfl(col[p], src, 64);
-              if (lane == src && p < kb) Xsb[sp * K * K + p * K + m] = cfs * xp;
-              if (live && p < kb) col[p] -= cfs * (xp * yq);
-            }
-            if (live) Ysb[sp * K * K + q * K + m] = yq;
-            if (lane == 0) {
-              cf[m] = cu;
-              cf[K + m] = cd;
-              aidx[m] = jj;
-              fld[b0 + jj] = (int8_t)(-s);
-            }
-            ++m;
+          for (int i = 0; i < R; ++i) {
+            const int p = h * R + i;
+            const double xp = __shfl(col[i], src, 64);
+            if (lane == src && p < kb) Xsb[sp * K * K + p * K + m] = cfs * xp;
+            if (live && p < kb) col[i] -= cfs * (xp * yq);
           }
+          if (live && h == 0) Ysb[sp * K * K + q * K + m] = yq;
+          if (lane == 0) {
+            cf[m] = cu;
+            cf[K + m] = cd;
+            aidx[m] = jj;
+            fld[b0 + jj] = (int8_t)(-s);
+          }
+          ++m;
         }
       }
       if (lane == 0) m_sh = m;
@@ -434,30 +496,53 @@ __global__ __launch_bounds__(1024) void k_flip_delayed(FlipArgs a) {
     const int m = m_sh;
     if (m == 0) continue;
     nacc += m;
+    // The coefficients of the two unit upper-triangular systems per spin, in accept
+    // order (r < p < m, else 0): upper T[r][p] = y_r(a_p) c_p, lower T[r][p] = -c_r x_r(a_p).
+    // They replace the block parts in place: matrix (spin s, lower) at Xsb + s 2K^2 + lower K^2.
+    constexpr int TE = 4 * K * K;
+    constexpr int TPT = (TE + 255) / 256;  // entries per thread (at least 256 threads)
+    {
+      double tv[TPT];
+#pragma unroll
+      for (int i = 0; i < TPT; ++i) {
+        const int e = threadIdx.x + i * blockDim.x;
+        double v = 0.0;
+        if (e < TE) {
+          const int ts = e / (2 * K * K), tl = (e / (K * K)) & 1, r = (e / K) % K, p = e % K;
+          if (r < p && p < m) {
+            const int ap = aidx[p];
+            v = tl ? -Xsb[ts * K * K + ap * K + r] : Ysb[ts * K * K + ap * K + r] * cf[ts * K + p];
+          }
+        }
+        tv[i] = v;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < TPT; ++i) {
+        const int e = threadIdx.x + i * blockDim.x;
+        if (e < TE) Xsb[e] = tv[i];
+      }
+    }
+    __syncthreads();
     // unit upper-triangular inverses, one per wave: waves 0/1 spin 0 (I+UC)^-1 / (I-L)^-1,
-    // waves 2/3 spin 1; lane j keeps column j in registers
+    // waves 2/3 spin 1; lane j forms column j by back substitution in registers (the
+    // loop over rows stays rolled; inv is written through a select chain)
     double inv[K];
     const int isp = wave >> 1;
     const bool lower = (wave & 1) != 0;
     const int j = lane;
     if (wave < 4) {
-      const double* Xs = Xsb + isp * K * K;
-      const double* Ys = Ysb + isp * K * K;
-      const double* cs = cf + isp * K;
+      const double* Tw = Xsb + isp * 2 * K * K + (lower ? K * K : 0);
 #pragma unroll
-      for (int r = K - 1; r >= 0; --r) {
+      for (int p = 0; p < K; ++p) inv[p] = p == j ? 1.0 : 0.0;
+#pragma unroll 1
+      for (int r = K - 2; r >= 0; --r) {
         double v = 0.0;
-        if (r == j) v = 1.0;
-        else if (r < j && j < m) {
 #pragma unroll
-          for (int p = r + 1; p < K; ++p)
-            if (p <= j) {
-              const int ap = aidx[p];
-              const double t = lower ? -Xs[ap * K + r] : Ys[ap * K + r] * cs[p];
-              v -= t * inv[p];
-            }
-        }
-        inv[r] = v;
+        for (int p = 0; p < K; ++p) v -= Tw[r * K + p] * inv[p];
+#pragma unroll
+        for (int p = 0; p < K; ++p)
+          if (p == r && r < j) inv[p] = v;
       }
     }
     __syncthreads();
@@ -471,21 +556,13 @@ __global__ __launch_bounds__(1024) void k_flip_delayed(FlipArgs a) {
         if (r <= j) out[aidx[r] * K + j] = lower ? inv[r] : cs[r] * inv[r];
     }
     __syncthreads();
-    const int K4 = (m + 3) & ~3;
-    for (int sp = 0; sp < 2; ++sp) {
-      T* P = wsT + (size_t)sp * spT;
-      strip_products<T>(G + sp * n2, N, Np, b0, kb, m, K4, Xsb + sp * K * K, Ysb + sp * K * K, K, P, pld,
-                        P + (size_t)Np * pld);
-    }
-    __syncthreads();
-    for (int sp = 0; sp < 2; ++sp) {
-      T* P = wsT + (size_t)sp * spT;
-      rank_update<T>(G + sp * n2, N, Np, P, pld, P + (size_t)Np * pld, K4);
-    }
+    block_update<T, K, (K >= 16 ? 4 : 2)>(G, n2, N, Np, b0, kb, m, Xsb, Ysb, wsT);
     __syncthreads();
   }
   slice_epilogue<T>(a, ch, c, l, G, nacc, underrun, bad, red);
 }
+
+int flip_delayed_threads_limit(int k) { return flip_delayed_max_threads(k); }
 
 size_t flip_delayed_shmem(int nmax, int k, int threads, int fp32) {
   (void)nmax;
@@ -497,7 +574,7 @@ size_t flip_delayed_shmem(int nmax, int k, int threads, int fp32) {
 size_t flip_delayed_workspace(int n, int k, int fp32) {
   const size_t Np = (n + 15) & ~15;
   const size_t K16 = (k + 15) & ~15;
-  return 2 * (Np * (K16 + 1) + K16 * Np) * (fp32 ? 4 : 8);
+  return 2 * K16 * Np * (fp32 ? 4 : 8);
 }
 
 void launch_flip(int fp32, int delayed, const FlipArgs& a, int nchains, int threads, size_t shmem,

@@ -98,28 +98,58 @@ __device__ inline void qr_wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Column sums over the 64 lanes of a wavefront: lane l holds NB partial sums v[c]; on
+// return every lane holds the NB totals in out[].  Halving exchange: at offset 32 the
+// lanes with bit 5 set keep the upper half of their values and trade the lower half with
+// their partner (bit 5 clear keeps the lower half), then offset 16 halves again, ...,
+// so after log2(NB) steps lane l carries column l / (64 / NB) alone and the remaining
+// offsets reduce it fully.  NB - 1 + 6 - log2(NB) exchanges in all, instead of 6 NB for
+// NB separate reductions.  The totals meet in the LDS scratch red[NB].
+template <int NB>
+__device__ inline void qr_wave_colsum(double (&v)[NB], double (&out)[NB], double* red, int lane) {
+  int off = 32;
+#pragma unroll
+  for (int h = NB / 2; h >= 1; h /= 2) {
+    const bool upper = (lane & off) != 0;
+#pragma unroll
+    for (int i = 0; i < h; ++i) {
+      const double send = upper ? v[i] : v[h + i];
+      const double keep = upper ? v[h + i] : v[i];
+      v[i] = keep + __shfl_xor(send, off, 64);
+    }
+    off >>= 1;
+  }
+  for (; off >= 1; off >>= 1) v[0] += __shfl_xor(v[0], off, 64);
+  constexpr int span = 64 / NB;
+  if (lane % span == 0) red[lane / span] = v[0];
+  qr_wave_sync();
+#pragma unroll
+  for (int c = 0; c < NB; ++c) out[c] = red[c];
+  qr_wave_sync();
+}
+
 // Householder factorisation of the panel (rows j0..N-1, w <= NB columns, in LDS) by one
 // wavefront, with panel-local column pivoting: at step q the remaining panel column of
-// largest norm over rows j..N-1 is swapped into place (with its R entries above the
-// panel in A and its pivot index).  The norms of all panel columns and the reflector's
-// dot products with them are accumulated in one pass each and reduced across lanes
-// together.  On return pnl holds R (on/above the diagonal) and the reflectors below it.
+// largest norm over rows j..N-1 is swapped into place (and its pivot index with it).
+// The norms of all panel columns and the reflector's dot products with them are each
+// reduced in one exchange pass.  The column swaps are recorded in swp[] for the R
+// entries above the panel, which the caller permutes afterwards.  On return pnl holds R
+// (on/above the diagonal) and the reflectors below it.
 template <int NB>
-__device__ void qr_panel_wave(double* pnl, double* A, int N, int j0, int w, int* perm, double* taus,
-                              bool& zero_pivot) {
+__device__ void qr_panel_wave(double* pnl, int N, int j0, int w, int* perm, int* swp, double* taus,
+                              double* red, bool& zero_pivot) {
   const int lane = threadIdx.x & 63;
   for (int q = 0; q < w; ++q) {
     const int j = j0 + q;
-    double nrm[NB];
+    double part[NB], nrm[NB];
 #pragma unroll
-    for (int c = 0; c < NB; ++c) nrm[c] = 0.0;
+    for (int c = 0; c < NB; ++c) part[c] = 0.0;
     for (int r = j + lane; r < N; r += 64) {
       const double* row = pnl + r * NB;
 #pragma unroll
-      for (int c = 0; c < NB; ++c) nrm[c] += row[c] * row[c];
+      for (int c = 0; c < NB; ++c) part[c] += row[c] * row[c];
     }
-#pragma unroll
-    for (int c = 0; c < NB; ++c) nrm[c] = qr_wave_sum(nrm[c]);
+    qr_wave_colsum<NB>(part, nrm, red, lane);
     int p = q;
     double best = -1.0;
 #pragma unroll
@@ -131,17 +161,13 @@ __device__ void qr_panel_wave(double* pnl, double* A, int N, int j0, int w, int*
         pnl[r * NB + q] = pnl[r * NB + p];
         pnl[r * NB + p] = t0;
       }
-      for (int r = lane; r < j0; r += 64) {
-        const double t0 = A[(size_t)r * N + j0 + q];
-        A[(size_t)r * N + j0 + q] = A[(size_t)r * N + j0 + p];
-        A[(size_t)r * N + j0 + p] = t0;
-      }
       if (lane == 0) {
         const int tp = perm[j0 + q];
         perm[j0 + q] = perm[j0 + p];
         perm[j0 + p] = tp;
       }
     }
+    if (lane == 0) swp[q] = p;
     qr_wave_sync();
     const double alpha = pnl[j * NB + q];
     const double nrmv = sqrt(best > 0.0 ? best : 0.0);
@@ -156,23 +182,22 @@ __device__ void qr_panel_wave(double* pnl, double* A, int N, int j0, int w, int*
     }
     for (int r = j + 1 + lane; r < N; r += 64) pnl[r * NB + q] *= scal;
     qr_wave_sync();
-    double d[NB];
+    double dp[NB], d[NB];
 #pragma unroll
-    for (int c = 0; c < NB; ++c) d[c] = 0.0;
+    for (int c = 0; c < NB; ++c) dp[c] = 0.0;
     for (int r = j + lane; r < N; r += 64) {
       const double v = r == j ? 1.0 : pnl[r * NB + q];
       const double* row = pnl + r * NB;
 #pragma unroll
-      for (int c = 0; c < NB; ++c) d[c] += v * row[c];
+      for (int c = 0; c < NB; ++c) dp[c] += v * row[c];
     }
-#pragma unroll
-    for (int c = 0; c < NB; ++c) d[c] = qr_wave_sum(d[c]) * tau;
+    qr_wave_colsum<NB>(dp, d, red, lane);
     for (int r = j + lane; r < N; r += 64) {
       const double v = r == j ? 1.0 : pnl[r * NB + q];
       double* row = pnl + r * NB;
 #pragma unroll
       for (int c = 0; c < NB; ++c)
-        if (c > q && c < w) row[c] -= v * d[c];
+        if (c > q && c < w) row[c] -= v * (d[c] * tau);
     }
     qr_wave_sync();
     if (lane == 0) {
@@ -231,9 +256,12 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
   double* pnl = sm;                    // [N][nb]
   double* Tm = pnl + (size_t)N * nb;   // [nb][nb]
   double* Ssc = Tm + nb * nb;          // [16][16]
-  double* norms = Ssc + 256;           // [N]
+  double* red = Ssc + 256;             // [16]
+  double* norms = red + 16;            // [N]
   double* taus = norms + N;            // [N]
   int* perm = reinterpret_cast<int*>(taus + N);  // [N]
+  int* swp = perm + N;                 // [16] column swaps of the current panel
+  int* fp = swp + 16;                  // [16] resulting panel column order
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   bool zero_pivot = false, bad = false;
 
@@ -287,14 +315,33 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
     }
     __syncthreads();
     if (wave == 0) {
-      if (nb == 16) qr_panel_wave<16>(pnl, A, N, j0, w, perm, taus, zero_pivot);
-      else qr_panel_wave<8>(pnl, A, N, j0, w, perm, taus, zero_pivot);
+      if (nb == 16) qr_panel_wave<16>(pnl, N, j0, w, perm, swp, taus, red, zero_pivot);
+      else qr_panel_wave<8>(pnl, N, j0, w, perm, swp, taus, red, zero_pivot);
+      if (lane == 0) {
+        for (int q = 0; q < nb; ++q) fp[q] = q;
+        for (int q = 0; q < w; ++q) {
+          const int t0 = fp[q];
+          fp[q] = fp[swp[q]];
+          fp[swp[q]] = t0;
+        }
+      }
     }
     __syncthreads();
-    // R (upper part) and V (below the diagonal) back to A; pnl becomes the explicit V
+    // R (upper part) and V (below the diagonal) back to A; the R entries above the panel
+    // follow the panel's column swaps; pnl becomes the explicit V
     for (int e = tid; e < (N - j0) * w; e += QR_THREADS) {
       const int r = j0 + e / w, q = e % w;
       A[(size_t)r * N + j0 + q] = pnl[r * nb + q];
+    }
+    for (int r = tid; r < j0; r += QR_THREADS) {
+      double* Ar = A + (size_t)r * N + j0;
+      double t[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (q < w) t[q] = Ar[fp[q]];
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (q < w) Ar[q] = t[q];
     }
     __syncthreads();
     for (int e = tid; e < w * w; e += QR_THREADS) {
@@ -443,7 +490,7 @@ __global__ __launch_bounds__(QF_THREADS) void k_trsm_perm(const TrsmTask* __rest
 
 size_t qr_blocked_shmem(int nmax) {
   const size_t nb = nmax > 256 ? 8 : 16;
-  return sizeof(double) * (nmax * nb + nb * nb + 256 + 2 * (size_t)nmax) + sizeof(int) * nmax;
+  return sizeof(double) * (nmax * nb + nb * nb + 256 + 16 + 2 * (size_t)nmax) + sizeof(int) * (nmax + 32);
 }
 
 size_t qr_formq_shmem(int nmax) {

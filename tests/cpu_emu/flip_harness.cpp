@@ -57,6 +57,10 @@ int main(int argc, char** argv) {
   }
   std::memset(st.data(), 0, sizeof(ChainState) * nch);
   FlipArgs a{chains.data(), st.data(), t, measure, 1, 1, K, 0};
+  if (K > 1 && (threads > flip_delayed_threads_limit(K) || threads < 256)) {
+    std::fprintf(stderr, "threads above the delayed kernel's launch bound %d\n", flip_delayed_threads_limit(K));
+    return 2;
+  }
   const size_t lds = K > 1 ? flip_delayed_shmem(N, K, threads, fp32) : sizeof(double) * (4 * (size_t)N + threads);
   std::printf("N=%d L=%d K=%d threads=%d fp32=%d chains=%d t=%d lds=%zu\n", N, L, K, threads, fp32, nch, t, lds);
   std::fflush(stdout);

@@ -26,6 +26,7 @@ inline emu_dim3 blockDim, gridDim;
 inline char* emu_lds_buf = nullptr;
 inline std::barrier<>* emu_block_barrier = nullptr;
 #define LQMC_DYNAMIC_LDS(type, name) type* name = reinterpret_cast<type*>(emu_lds_buf)
+#define LQMC_GLOBAL
 inline void __syncthreads() { emu_block_barrier->arrive_and_wait(); }
 
 using std::fabs; using std::fmax; using std::fmin; using std::isfinite; using std::max; using std::min; using std::sqrt;
@@ -46,6 +47,7 @@ inline unsigned long long atomicMax(unsigned long long* p, unsigned long long v)
 }
 inline uint32_t __umulhi(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
 inline long long __double_as_longlong(double d) { long long r; std::memcpy(&r, &d, 8); return r; }
+inline double __longlong_as_double(long long v) { double r; std::memcpy(&r, &v, 8); return r; }
 
 struct EmuWave { double a[64], b[64]; std::barrier<>* bar; };
 inline EmuWave* emu_waves = nullptr;
@@ -99,6 +101,10 @@ inline double __shfl(double v, int src, int width = 64) {
   s.bar->arrive_and_wait();
   return r;
 }
+
+// Scalar broadcast of lane l's value (l uniform across the wavefront).
+inline int emu_readlane_i32(int v, int l) { return (int)(long long)__shfl((double)v, l); }
+#define __builtin_amdgcn_readlane(v, l) emu_readlane_i32((v), (l))
 
 // Run fn() as a block of `threads` threads for every block index in [0, blocks).
 template <typename F>

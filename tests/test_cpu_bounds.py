@@ -41,7 +41,7 @@ def harness_tsan(tmp_path_factory):
     "36 20 32 256 0 2 3 0 40",      # delayed, rejections, compared with rank-1
     "100 60 8 256 0 2 59 1 40",     # ragged last block, measurement slice
     "64 40 8 256 1 2 39 1 30",      # fp32
-    "256 80 32 1024 0 1 0 0 60",    # the throughput configuration
+    "256 80 32 512 0 1 0 0 60",     # the throughput configuration
 ])
 def test_flip_kernel_in_bounds(harness, args):
     """In bounds under ASan; delayed runs (9 args) must match the rank-1 kernel on
@@ -54,6 +54,7 @@ def test_flip_kernel_in_bounds(harness, args):
 @pytest.mark.parametrize("args", [
     "64 16 8 256 0 1 0 1",      # delayed fp64, several waves
     "64 16 8 256 1 1 15 1",     # delayed fp32
+    "64 16 16 512 0 1 15 1",    # delayed fp64, 16-site blocks on the 512-thread bound
     "36 20 32 256 0 1 3 0",
     "64 16 1 256 0 1 0 1",      # rank-1
 ])
