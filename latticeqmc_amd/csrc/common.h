@@ -15,6 +15,19 @@ __device__ inline LQMC_GLOBAL T* gptr(T* p) {
   return (LQMC_GLOBAL T*)p;
 }
 
+// p[ok ? i : 0], then 0 where !ok: the load is unconditional (no branch, so loads of
+// a loop body can all be in flight together) and always addresses valid memory.
+template <typename T>
+__device__ inline T ld0(const T* p, size_t i, bool ok) {
+  const T v = p[ok ? i : 0];
+  return ok ? v : T(0);
+}
+template <typename T>
+__device__ inline T ldg0(const LQMC_GLOBAL T* p, size_t i, bool ok) {
+  const T v = p[ok ? i : 0];
+  return ok ? v : T(0);
+}
+
 namespace lqmc {
 
 // Static description of one Markov chain (read-only on the device).

@@ -22,19 +22,6 @@ namespace lqmc {
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-// p[ok ? i : 0], then 0 where !ok: the load is unconditional (no branch, so loads of
-// a loop body can all be in flight together) and always addresses valid memory.
-template <typename T>
-__device__ inline T ld0(const T* p, size_t i, bool ok) {
-  const T v = p[ok ? i : 0];
-  return ok ? v : T(0);
-}
-template <typename T>
-__device__ inline T ldg0(const LQMC_GLOBAL T* p, size_t i, bool ok) {
-  const T v = p[ok ? i : 0];
-  return ok ? v : T(0);
-}
-
 // Value of lane l (wavefront-uniform) broadcast through scalar registers.
 __device__ inline double readlane_f64(double v, int l) {
   const unsigned long long b = (unsigned long long)__double_as_longlong(v);
@@ -94,8 +81,9 @@ __device__ void slice_epilogue(const FlipArgs& a, const ChainDev& ch, int c, int
   __syncthreads();
   for (int i = threadIdx.x; i < N; i += blockDim.x) {
     const int s = ch.field[(size_t)l * N + i];
-    const double up = s > 0 ? ch.e1m : ch.e1p;
-    const double dn = s > 0 ? ch.e1p : ch.e1m;
+    const double e1m = ch.e1m, e1p = ch.e1p;  // values, so the conditionals select values
+    const double up = s > 0 ? e1m : e1p;
+    const double dn = s > 0 ? e1p : e1m;
     ch.ev[(size_t)l * N + i] = up;
     ch.ev[(size_t)NL + (size_t)l * N + i] = dn;
     ch.evi[(size_t)l * N + i] = 1.0 / up;
@@ -167,8 +155,9 @@ __global__ __launch_bounds__(1024) void k_flip_rank1(FlipArgs a) {
     const int s = fld[i];
     const double gu = (double)G[(size_t)i * N + i];
     const double gd = (double)G[n2 + (size_t)i * N + i];
-    const double ep = s > 0 ? ch.e2p : ch.e2m;  // exp(+2 lambda s)
-    const double em = s > 0 ? ch.e2m : ch.e2p;  // exp(-2 lambda s)
+    const double e2p = ch.e2p, e2m = ch.e2m;
+    const double ep = s > 0 ? e2p : e2m;  // exp(+2 lambda s)
+    const double em = s > 0 ? e2m : e2p;  // exp(-2 lambda s)
     const double d_up = 1.0 + (1.0 - gu) * (ep - 1.0);
     const double d_dn = 1.0 + (1.0 - gd) * (em - 1.0);
     const double ratio = d_up * d_dn;
@@ -455,8 +444,9 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
         const double gu = readlane_f64(cj, hj * K + jj);
         const double gd = readlane_f64(cj, K * H + hj * K + jj);
         const int s = sb[jj];
-        const double ep = s > 0 ? ch.e2p : ch.e2m;
-        const double em = s > 0 ? ch.e2m : ch.e2p;
+        const double e2p = ch.e2p, e2m = ch.e2m;
+        const double ep = s > 0 ? e2p : e2m;
+        const double em = s > 0 ? e2m : e2p;
         const double d_up = 1.0 + (1.0 - gu) * (ep - 1.0);
         const double d_dn = 1.0 + (1.0 - gd) * (em - 1.0);
         const double ratio = d_up * d_dn;

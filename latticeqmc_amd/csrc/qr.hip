@@ -38,33 +38,35 @@ __device__ inline qr_v4d qr_mma(double a, double b, qr_v4d c) {
 }
 
 // X[r0:N, c_begin:c_end] <- (I - V T' V^T) X[r0:N, c_begin:c_end] on the FP64 matrix
-// cores, V = pnl (explicit reflectors, rows r0..N-1, w <= 16 columns), T' = T^T (trans)
-// or T.  One wavefront per 16-column tile (NW wavefronts in the workgroup):
-// W = V^T X (K = rows), W' = T' W, X -= V W'.  In the 16x16x4 FP64 layout the
-// accumulator element v of lane l is row l/16 + 4v, column l%16 -- exactly the B operand
-// lane l supplies for k-step v -- so W and W' chain through registers.  Ends with a
-// workgroup barrier.
+// cores, V = pnl (explicit reflectors, rows r0..N-1, w <= 16 columns, row stride ps),
+// T' = T^T (trans) or T (w x w, leading dimension nb).  One wavefront per 16-column tile
+// (NW wavefronts in the workgroup): W = V^T X (K = rows), W' = T' W, X -= V W'.  In the
+// 16x16x4 FP64 layout the accumulator element v of lane l is row l/16 + 4v, column
+// l%16 -- exactly the B operand lane l supplies for k-step v -- so W and W' chain
+// through registers.  Loads are unconditional (clamped), so a loop's loads overlap.
+// Ends with a workgroup barrier.
 template <int NW>
-__device__ void qr_apply_block(double* X, int N, int r0, int c_begin, int c_end, const double* pnl, int nb,
-                               int w, const double* Tm, bool trans) {
+__device__ void qr_apply_block(double* X, int N, int r0, int c_begin, int c_end, const double* pnl, int ps,
+                               int w, const double* Tm, int nb, bool trans) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int nct = (c_end - c_begin + 15) >> 4;
+  LQMC_GLOBAL double* Xg = gptr(X);
   for (int ct = wave; ct < nct; ct += NW) {
     const int c = c_begin + ct * 16 + li;
     const bool cok = c < c_end;
     qr_v4d wt = {0.0, 0.0, 0.0, 0.0};
     for (int kk = r0; kk < N; kk += 4) {
       const int r = kk + lk;
-      const double a = (r < N && li < w) ? pnl[r * nb + li] : 0.0;
-      const double b = (r < N && cok) ? X[(size_t)r * N + c] : 0.0;
+      const double a = ld0(pnl, (size_t)r * ps + li, r < N && li < w);
+      const double b = ldg0(Xg, (size_t)r * N + c, r < N && cok);
       wt = qr_mma(a, b, wt);
     }
     qr_v4d w2 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const int p = 4 * s + lk;
-      const double a = (li < w && p < w) ? (trans ? Tm[p * nb + li] : Tm[li * nb + p]) : 0.0;
+      const double a = ld0(Tm, trans ? (size_t)p * nb + li : (size_t)li * nb + p, li < w && p < w);
       w2 = qr_mma(a, wt[s], w2);
     }
     for (int rt = r0; rt < N; rt += 16) {
@@ -72,19 +74,19 @@ __device__ void qr_apply_block(double* X, int N, int r0, int c_begin, int c_end,
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int row = rt + lk + 4 * v;
-        acc[v] = (row < N && cok) ? X[(size_t)row * N + c] : 0.0;
+        acc[v] = ldg0(Xg, (size_t)row * N + c, row < N && cok);
       }
       const int ra = rt + li;
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         const int q = 4 * s + lk;
-        const double a = (ra < N && q < w) ? -pnl[ra * nb + q] : 0.0;
+        const double a = -ld0(pnl, (size_t)ra * ps + q, ra < N && q < w);
         acc = qr_mma(a, w2[s], acc);
       }
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int row = rt + lk + 4 * v;
-        if (row < N && cok) X[(size_t)row * N + c] = acc[v];
+        if (row < N && cok) Xg[(size_t)row * N + c] = acc[v];
       }
     }
   }
@@ -99,36 +101,43 @@ __device__ inline void qr_wave_sync() {
 }
 
 // Column sums over the 64 lanes of a wavefront: lane l holds NB partial sums v[c]; on
-// return every lane holds the NB totals in out[].  Halving exchange: at offset 32 the
-// lanes with bit 5 set keep the upper half of their values and trade the lower half with
-// their partner (bit 5 clear keeps the lower half), then offset 16 halves again, ...,
-// so after log2(NB) steps lane l carries column l / (64 / NB) alone and the remaining
-// offsets reduce it fully.  NB - 1 + 6 - log2(NB) exchanges in all, instead of 6 NB for
-// NB separate reductions.  The totals meet in the LDS scratch red[NB].
+// return the NB totals are in the LDS scratch red[0..NB) (visible to the whole
+// wavefront).  Halving exchange: at offset 32 the lanes with bit 5 set keep the upper
+// half of their values and trade the lower half with their partner (bit 5 clear keeps
+// the lower half), then offset 16 halves again, ..., so after log2(NB) steps lane l
+// carries column l / (64 / NB) alone and the remaining offsets reduce it fully.
+// NB - 1 + 6 - log2(NB) exchanges in all, instead of 6 NB for NB separate reductions.
 template <int NB>
-__device__ inline void qr_wave_colsum(double (&v)[NB], double (&out)[NB], double* red, int lane) {
-  int off = 32;
+__device__ inline void qr_wave_colsum(double (&v)[NB], double* red, int lane) {
+  constexpr int LG = NB >= 16 ? 4 : NB >= 8 ? 3 : NB >= 4 ? 2 : NB >= 2 ? 1 : 0;
+  static_assert((1 << LG) == NB, "NB must be a power of two <= 16");
 #pragma unroll
-  for (int h = NB / 2; h >= 1; h /= 2) {
+  for (int st = 0; st < LG; ++st) {  // linear counter: fully unrolled, v stays in registers
+    const int h = NB >> (st + 1);
+    const int off = 32 >> st;
     const bool upper = (lane & off) != 0;
 #pragma unroll
-    for (int i = 0; i < h; ++i) {
-      const double send = upper ? v[i] : v[h + i];
-      const double keep = upper ? v[h + i] : v[i];
-      v[i] = keep + __shfl_xor(send, off, 64);
+    for (int i = 0; i < NB / 2; ++i) {
+      if (i < h) {
+        // read both halves into values first: a conditional over the array elements
+        // themselves would select between their addresses and push v out to scratch
+        const double lo = v[i], hi = v[h + i];
+        const double send = upper ? lo : hi;
+        const double keep = upper ? hi : lo;
+        v[i] = keep + __shfl_xor(send, off, 64);
+      }
     }
-    off >>= 1;
   }
-  for (; off >= 1; off >>= 1) v[0] += __shfl_xor(v[0], off, 64);
-  constexpr int span = 64 / NB;
-  if (lane % span == 0) red[lane / span] = v[0];
-  qr_wave_sync();
 #pragma unroll
-  for (int c = 0; c < NB; ++c) out[c] = red[c];
+  for (int st = LG; st < 6; ++st) v[0] += __shfl_xor(v[0], 32 >> st, 64);
+  constexpr int span = 64 / NB;
+  qr_wave_sync();  // earlier readers of red are done
+  if (lane % span == 0) red[lane / span] = v[0];
   qr_wave_sync();
 }
 
-// Householder factorisation of the panel (rows j0..N-1, w <= NB columns, in LDS) by one
+// Householder factorisation of the panel (rows j0..N-1, w <= NB columns, in LDS with row
+// stride NB + 1) by one
 // wavefront, with panel-local column pivoting: at step q the remaining panel column of
 // largest norm over rows j..N-1 is swapped into place (and its pivot index with it).
 // The norms of all panel columns and the reflector's dot products with them are each
@@ -139,27 +148,30 @@ template <int NB>
 __device__ void qr_panel_wave(double* pnl, int N, int j0, int w, int* perm, int* swp, double* taus,
                               double* red, bool& zero_pivot) {
   const int lane = threadIdx.x & 63;
+  constexpr int PS = NB + 1;  // padded row stride: lanes walking rows spread over the banks
   for (int q = 0; q < w; ++q) {
     const int j = j0 + q;
-    double part[NB], nrm[NB];
+    double part[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c) part[c] = 0.0;
     for (int r = j + lane; r < N; r += 64) {
-      const double* row = pnl + r * NB;
+      const double* row = pnl + r * PS;
 #pragma unroll
       for (int c = 0; c < NB; ++c) part[c] += row[c] * row[c];
     }
-    qr_wave_colsum<NB>(part, nrm, red, lane);
+    qr_wave_colsum<NB>(part, red, lane);
     int p = q;
     double best = -1.0;
 #pragma unroll
-    for (int c = 0; c < NB; ++c)
-      if (c >= q && c < w && nrm[c] > best) { best = nrm[c]; p = c; }
+    for (int c = 0; c < NB; ++c) {  // static trip count: the NB loads are issued together
+      const double nc = red[c];
+      if (c >= q && c < w && nc > best) { best = nc; p = c; }
+    }
     if (p != q) {
       for (int r = j0 + lane; r < N; r += 64) {
-        const double t0 = pnl[r * NB + q];
-        pnl[r * NB + q] = pnl[r * NB + p];
-        pnl[r * NB + p] = t0;
+        const double t0 = pnl[r * PS + q];
+        pnl[r * PS + q] = pnl[r * PS + p];
+        pnl[r * PS + p] = t0;
       }
       if (lane == 0) {
         const int tp = perm[j0 + q];
@@ -169,7 +181,7 @@ __device__ void qr_panel_wave(double* pnl, int N, int j0, int w, int* perm, int*
     }
     if (lane == 0) swp[q] = p;
     qr_wave_sync();
-    const double alpha = pnl[j * NB + q];
+    const double alpha = pnl[j * PS + q];
     const double nrmv = sqrt(best > 0.0 ? best : 0.0);
     double beta, tau, scal;
     if (nrmv == 0.0) {
@@ -180,28 +192,30 @@ __device__ void qr_panel_wave(double* pnl, int N, int j0, int w, int* perm, int*
       tau = (beta - alpha) / beta;
       scal = 1.0 / (alpha - beta);
     }
-    for (int r = j + 1 + lane; r < N; r += 64) pnl[r * NB + q] *= scal;
+    for (int r = j + 1 + lane; r < N; r += 64) pnl[r * PS + q] *= scal;
     qr_wave_sync();
-    double dp[NB], d[NB];
+    double dp[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c) dp[c] = 0.0;
     for (int r = j + lane; r < N; r += 64) {
-      const double v = r == j ? 1.0 : pnl[r * NB + q];
-      const double* row = pnl + r * NB;
+      const double v = r == j ? 1.0 : pnl[r * PS + q];
+      const double* row = pnl + r * PS;
 #pragma unroll
       for (int c = 0; c < NB; ++c) dp[c] += v * row[c];
     }
-    qr_wave_colsum<NB>(dp, d, red, lane);
+    qr_wave_colsum<NB>(dp, red, lane);
+#pragma unroll
+    for (int c = 0; c < NB; ++c) dp[c] = (c > q && c < w) ? red[c] * tau : 0.0;
     for (int r = j + lane; r < N; r += 64) {
-      const double v = r == j ? 1.0 : pnl[r * NB + q];
-      double* row = pnl + r * NB;
+      const double v = r == j ? 1.0 : pnl[r * PS + q];
+      double* row = pnl + r * PS;
 #pragma unroll
       for (int c = 0; c < NB; ++c)
-        if (c > q && c < w) row[c] -= v * (d[c] * tau);
+        if (c > q && c < w) row[c] -= v * dp[c];
     }
     qr_wave_sync();
     if (lane == 0) {
-      pnl[j * NB + q] = beta;
+      pnl[j * PS + q] = beta;
       taus[j] = tau;
     }
     qr_wave_sync();
@@ -209,19 +223,19 @@ __device__ void qr_panel_wave(double* pnl, int N, int j0, int w, int* perm, int*
 }
 
 // T (w x w, leading dimension nb) of the compact-WY form I - V T V^T = H_0 ... H_{w-1}
-// of a panel's reflectors (V explicit in pnl, rows r0..N-1).  Wavefront 0 forms
+// of a panel's reflectors (V explicit in pnl, rows r0..N-1, row stride ps).  Wavefront 0 forms
 // S = V^T V on the matrix cores (A and B operands coincide), then lane p computes row p
 // of T in registers from the recurrence T[p][p] = tau_p,
 // T[p][i] = -tau_i * sum_{p <= q < i} T[p][q] S[q][i]  (rows are independent).
 // S is a 16 x 16 LDS scratch.  Ends with a workgroup barrier.
-__device__ void qr_build_t(const double* pnl, int nb, int r0, int N, int w, const double* tau, double* Tm,
-                           double* S) {
+__device__ void qr_build_t(const double* pnl, int ps, int nb, int r0, int N, int w, const double* tau,
+                           double* Tm, double* S) {
   if (threadIdx.x < 64) {
     const int lane = threadIdx.x, li = lane & 15, lk = lane >> 4;
     qr_v4d acc = {0.0, 0.0, 0.0, 0.0};
     for (int kk = r0; kk < N; kk += 4) {
       const int r = kk + lk;
-      const double v = (r < N && li < w) ? pnl[r * nb + li] : 0.0;
+      const double v = ld0(pnl, (size_t)r * ps + li, r < N && li < w);
       acc = qr_mma(v, v, acc);
     }
 #pragma unroll
@@ -250,11 +264,12 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
   const QrTask tk = tasks[blockIdx.x];
   const int N = tk.N;
   const int nb = qr_panel_width(N);
+  const int ps = nb + 1;               // panel row stride (padded)
   double* __restrict__ W = tk.W;
   double* __restrict__ A = tk.A;
   LQMC_DYNAMIC_LDS(double, sm);
-  double* pnl = sm;                    // [N][nb]
-  double* Tm = pnl + (size_t)N * nb;   // [nb][nb]
+  double* pnl = sm;                    // [N][ps]
+  double* Tm = pnl + (size_t)N * ps;   // [nb][nb]
   double* Ssc = Tm + nb * nb;          // [16][16]
   double* red = Ssc + 256;             // [16]
   double* norms = red + 16;            // [N]
@@ -311,7 +326,7 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
     const int w = min(nb, N - j0);
     for (int e = tid; e < (N - j0) * nb; e += QR_THREADS) {
       const int r = j0 + e / nb, q = e % nb;
-      pnl[r * nb + q] = q < w ? A[(size_t)r * N + j0 + q] : 0.0;
+      pnl[r * ps + q] = q < w ? A[(size_t)r * N + j0 + q] : 0.0;
     }
     __syncthreads();
     if (wave == 0) {
@@ -331,7 +346,7 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
     // follow the panel's column swaps; pnl becomes the explicit V
     for (int e = tid; e < (N - j0) * w; e += QR_THREADS) {
       const int r = j0 + e / w, q = e % w;
-      A[(size_t)r * N + j0 + q] = pnl[r * nb + q];
+      A[(size_t)r * N + j0 + q] = pnl[r * ps + q];
     }
     for (int r = tid; r < j0; r += QR_THREADS) {
       double* Ar = A + (size_t)r * N + j0;
@@ -346,17 +361,17 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
     __syncthreads();
     for (int e = tid; e < w * w; e += QR_THREADS) {
       const int rr = e / w, q = e % w;
-      if (rr < q) pnl[(j0 + rr) * nb + q] = 0.0;
-      else if (rr == q) pnl[(j0 + rr) * nb + q] = 1.0;
+      if (rr < q) pnl[(j0 + rr) * ps + q] = 0.0;
+      else if (rr == q) pnl[(j0 + rr) * ps + q] = 1.0;
     }
     __syncthreads();
-    qr_build_t(pnl, nb, j0, N, w, taus + j0, Tm, Ssc);
+    qr_build_t(pnl, ps, nb, j0, N, w, taus + j0, Tm, Ssc);
     double* tout = tk.Tt + (size_t)(j0 / nb) * nb * nb;
     for (int e = tid; e < nb * nb; e += QR_THREADS) {
       const int p = e / nb, q = e % nb;
       tout[e] = (p < w && q < w) ? Tm[e] : 0.0;
     }
-    if (j0 + w < N) qr_apply_block<QR_WAVES>(A, N, j0, j0 + w, N, pnl, nb, w, Tm, true);
+    if (j0 + w < N) qr_apply_block<QR_WAVES>(A, N, j0, j0 + w, N, pnl, ps, w, Tm, nb, true);
   }
 
   // D = |diag R|; V'' = D^-1 R P^T over W, or R' = D^-1 R (unpermuted) with the pivot order
@@ -394,11 +409,12 @@ __global__ __launch_bounds__(QF_THREADS) void k_qr_formq(const QrTask* __restric
   if (c0 >= N) return;
   const int c1 = min(N, c0 + QF_COLS), cw = c1 - c0;
   const int nb = qr_panel_width(N);
+  const int ps = nb + 1;
   const double* __restrict__ A = tk.A;
   double* __restrict__ Q = tk.Q;
   LQMC_DYNAMIC_LDS(double, sm);
-  double* pnl = sm;                    // [N][nb]
-  double* Tm = pnl + (size_t)N * nb;   // [nb][nb]
+  double* pnl = sm;                    // [N][ps]
+  double* Tm = pnl + (size_t)N * ps;   // [nb][nb]
   const int tid = threadIdx.x;
   for (int e = tid; e < N * cw; e += QF_THREADS) {
     const int r = e / cw, c = c0 + e % cw;
@@ -411,12 +427,12 @@ __global__ __launch_bounds__(QF_THREADS) void k_qr_formq(const QrTask* __restric
       const int rr = r - j0;
       double v = 0.0;
       if (q < w) v = rr < q ? 0.0 : (rr == q ? 1.0 : A[(size_t)r * N + j0 + q]);
-      pnl[r * nb + q] = v;
+      pnl[r * ps + q] = v;
     }
     const double* tin = tk.Tt + (size_t)(j0 / nb) * nb * nb;
     for (int e = tid; e < nb * nb; e += QF_THREADS) Tm[e] = tin[e];
     __syncthreads();
-    qr_apply_block<QF_THREADS / 64>(Q, N, j0, max(c0, j0), c1, pnl, nb, w, Tm, false);
+    qr_apply_block<QF_THREADS / 64>(Q, N, j0, max(c0, j0), c1, pnl, ps, w, Tm, nb, false);
   }
 }
 
@@ -433,6 +449,8 @@ __global__ __launch_bounds__(QF_THREADS) void k_trsm_perm(const TrsmTask* __rest
   const int c1 = min(N, c0 + QF_COLS);
   const double* __restrict__ R = tk.V;
   double* __restrict__ Z = tk.Z;
+  const LQMC_GLOBAL double* Rg = gptr(R);
+  const LQMC_GLOBAL double* Zg = gptr(Z);
   LQMC_DYNAMIC_LDS(double, sm);
   double* blk = sm;                     // [TRSM_B][QF_COLS]
   double* rb = blk + TRSM_B * QF_COLS;  // [TRSM_B][TRSM_B]
@@ -456,8 +474,8 @@ __global__ __launch_bounds__(QF_THREADS) void k_trsm_perm(const TrsmTask* __rest
     const int ra = i0 + li;
     for (int k = i0 + b; k < N; k += 4) {
       const int kr = k + lk;
-      const double a = (li < b && kr < N) ? -R[(size_t)ra * N + kr] : 0.0;
-      const double y = (kr < N && cok) ? Z[(size_t)kr * N + c] : 0.0;
+      const double a = -ldg0(Rg, (size_t)ra * N + kr, li < b && kr < N);
+      const double y = ldg0(Zg, (size_t)kr * N + c, kr < N && cok);
       acc = qr_mma(a, y, acc);
     }
 #pragma unroll
@@ -490,12 +508,12 @@ __global__ __launch_bounds__(QF_THREADS) void k_trsm_perm(const TrsmTask* __rest
 
 size_t qr_blocked_shmem(int nmax) {
   const size_t nb = nmax > 256 ? 8 : 16;
-  return sizeof(double) * (nmax * nb + nb * nb + 256 + 16 + 2 * (size_t)nmax) + sizeof(int) * (nmax + 32);
+  return sizeof(double) * (nmax * (nb + 1) + nb * nb + 256 + 16 + 2 * (size_t)nmax) + sizeof(int) * (nmax + 32);
 }
 
 size_t qr_formq_shmem(int nmax) {
   const size_t nb = nmax > 256 ? 8 : 16;
-  return sizeof(double) * (nmax * nb + nb * nb);
+  return sizeof(double) * (nmax * (nb + 1) + nb * nb);
 }
 
 size_t trsm_shmem(int) { return sizeof(double) * (TRSM_B * QF_COLS + TRSM_B * TRSM_B); }
