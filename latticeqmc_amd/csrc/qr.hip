@@ -93,6 +93,11 @@ __device__ void qr_apply_block(double* X, int N, int r0, int c_begin, int c_end,
   __syncthreads();
 }
 
+// Keeps the instruction scheduler from moving code across this point: the loads issued
+// before it are all in flight before the arithmetic after it starts (left to itself the
+// scheduler interleaves each LDS load with its use and waits on every one).
+__device__ inline void qr_sched_fence() { __builtin_amdgcn_sched_barrier(0); }
+
 // Orders LDS traffic between the lanes of one wavefront.
 __device__ inline void qr_wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -100,39 +105,57 @@ __device__ inline void qr_wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Column sums over the 64 lanes of a wavefront: lane l holds NB partial sums v[c]; on
-// return the NB totals are in the LDS scratch red[0..NB) (visible to the whole
-// wavefront).  Halving exchange: at offset 32 the lanes with bit 5 set keep the upper
-// half of their values and trade the lower half with their partner (bit 5 clear keeps
-// the lower half), then offset 16 halves again, ..., so after log2(NB) steps lane l
-// carries column l / (64 / NB) alone and the remaining offsets reduce it fully.
-// NB - 1 + 6 - log2(NB) exchanges in all, instead of 6 NB for NB separate reductions.
+// Lane exchange inside 16-lane rows through the DPP network (a VALU operation, no LDS
+// round trip); CTRL is the GFX9 dpp_ctrl encoding, applied to both halves of a double.
+template <int CTRL>
+__device__ inline double qr_dpp(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xF, 0xF, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Column sums over the 64 lanes of a wavefront: lane l holds NB (8 or 16) partial sums
+// v[c]; on return the NB totals are in the LDS scratch red[0..NB).  Two halving
+// exchanges across the 16-lane rows (offset 32: lanes with bit 5 set keep the upper half
+// of their values and trade the lower half with their partner; then offset 16 on the
+// remaining half) leave NB/4 columns per lane, (NB/2) b5 + (NB/4) b4 + i; each is then
+// summed over its 16-lane row by four DPP steps (swap within pairs, within quads, rotate
+// the row by 4, by 8).  Reading both halves into values before the conditional keeps v
+// in registers (a conditional over the elements would select their addresses).
 template <int NB>
 __device__ inline void qr_wave_colsum(double (&v)[NB], double* red, int lane) {
-  constexpr int LG = NB >= 16 ? 4 : NB >= 8 ? 3 : NB >= 4 ? 2 : NB >= 2 ? 1 : 0;
-  static_assert((1 << LG) == NB, "NB must be a power of two <= 16");
-#pragma unroll
-  for (int st = 0; st < LG; ++st) {  // linear counter: fully unrolled, v stays in registers
-    const int h = NB >> (st + 1);
-    const int off = 32 >> st;
-    const bool upper = (lane & off) != 0;
+  static_assert(NB == 8 || NB == 16, "panel width");
+  {
+    const bool up = (lane & 32) != 0;
 #pragma unroll
     for (int i = 0; i < NB / 2; ++i) {
-      if (i < h) {
-        // read both halves into values first: a conditional over the array elements
-        // themselves would select between their addresses and push v out to scratch
-        const double lo = v[i], hi = v[h + i];
-        const double send = upper ? lo : hi;
-        const double keep = upper ? hi : lo;
-        v[i] = keep + __shfl_xor(send, off, 64);
-      }
+      const double lo = v[i], hi = v[NB / 2 + i];
+      v[i] = (up ? hi : lo) + __shfl_xor(up ? lo : hi, 32, 64);
+    }
+  }
+  {
+    const bool up = (lane & 16) != 0;
+#pragma unroll
+    for (int i = 0; i < NB / 4; ++i) {
+      const double lo = v[i], hi = v[NB / 4 + i];
+      v[i] = (up ? hi : lo) + __shfl_xor(up ? lo : hi, 16, 64);
     }
   }
 #pragma unroll
-  for (int st = LG; st < 6; ++st) v[0] += __shfl_xor(v[0], 32 >> st, 64);
-  constexpr int span = 64 / NB;
+  for (int i = 0; i < NB / 4; ++i) {
+    double x = v[i];
+    x += qr_dpp<0xB1>(x);   // quad_perm [1,0,3,2]
+    x += qr_dpp<0x4E>(x);   // quad_perm [2,3,0,1]
+    x += qr_dpp<0x124>(x);  // row_ror:4
+    x += qr_dpp<0x128>(x);  // row_ror:8
+    v[i] = x;
+  }
   qr_wave_sync();  // earlier readers of red are done
-  if (lane % span == 0) red[lane / span] = v[0];
+  if ((lane & 15) == 0) {
+#pragma unroll
+    for (int i = 0; i < NB / 4; ++i) red[(NB / 2) * (lane >> 5) + (NB / 4) * ((lane >> 4) & 1) + i] = v[i];
+  }
   qr_wave_sync();
 }
 
@@ -157,7 +180,14 @@ __device__ void qr_panel_wave(double* pnl, int N, int j0, int w, int* perm, int*
     for (int r = j + lane; r < N; r += 64) {
       const double* row = pnl + r * PS;
 #pragma unroll
-      for (int c = 0; c < NB; ++c) part[c] += row[c] * row[c];
+      for (int c0 = 0; c0 < NB; c0 += 8) {  // batches of 8 loads in flight
+        double rv[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) rv[c] = row[c0 + c];
+        qr_sched_fence();
+#pragma unroll
+        for (int c = 0; c < 8; ++c) part[c0 + c] += rv[c] * rv[c];
+      }
     }
     qr_wave_colsum<NB>(part, red, lane);
     int p = q;
@@ -198,20 +228,33 @@ __device__ void qr_panel_wave(double* pnl, int N, int j0, int w, int* perm, int*
 #pragma unroll
     for (int c = 0; c < NB; ++c) dp[c] = 0.0;
     for (int r = j + lane; r < N; r += 64) {
-      const double v = r == j ? 1.0 : pnl[r * PS + q];
       const double* row = pnl + r * PS;
+      const double v = r == j ? 1.0 : row[q];
 #pragma unroll
-      for (int c = 0; c < NB; ++c) dp[c] += v * row[c];
+      for (int c0 = 0; c0 < NB; c0 += 8) {
+        double rv[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) rv[c] = row[c0 + c];
+        qr_sched_fence();
+#pragma unroll
+        for (int c = 0; c < 8; ++c) dp[c0 + c] += v * rv[c];
+      }
     }
     qr_wave_colsum<NB>(dp, red, lane);
 #pragma unroll
     for (int c = 0; c < NB; ++c) dp[c] = (c > q && c < w) ? red[c] * tau : 0.0;
     for (int r = j + lane; r < N; r += 64) {
-      const double v = r == j ? 1.0 : pnl[r * PS + q];
       double* row = pnl + r * PS;
+      const double v = r == j ? 1.0 : row[q];
 #pragma unroll
-      for (int c = 0; c < NB; ++c)
-        if (c > q && c < w) row[c] -= v * dp[c];
+      for (int c0 = 0; c0 < NB; c0 += 8) {
+        double rv[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) rv[c] = row[c0 + c];
+        qr_sched_fence();
+#pragma unroll
+        for (int c = 0; c < 8; ++c) row[c0 + c] = rv[c] - v * dp[c0 + c];  // dp = 0 off (q, w)
+      }
     }
     qr_wave_sync();
     if (lane == 0) {
@@ -284,10 +327,10 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
   // column within a 64-column tile, the wavefronts of a tile split the rows; partial
   // sums meet in the (not yet used) panel buffer.
   {
-    const int nct = (N + 63) >> 6;            // <= 16 for N <= 1024
-    const int ng = QR_WAVES / nct;            // row groups per tile
-    const int ct = wave % nct, g = wave / nct;
-    if (g < ng) {
+    const int nct = (N + 63) >> 6;                       // 64-column tiles
+    const int ng = QR_WAVES > nct ? QR_WAVES / nct : 1;  // row groups per tile
+    for (int t = wave; t < nct * ng; t += QR_WAVES) {
+      const int ct = t % nct, g = t / nct;
       const int c = ct * 64 + lane;
       double s = 0.0;
       if (c < N)

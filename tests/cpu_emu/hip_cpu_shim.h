@@ -8,6 +8,7 @@
 #include <barrier>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -77,6 +78,7 @@ inline V emu_mfma(T a, T b, V c, bool fp64) {
 inline void emu_wave_sync() { emu_waves[threadIdx.x >> 6].bar->arrive_and_wait(); }
 #define __builtin_amdgcn_wave_barrier() emu_wave_sync()
 #define __builtin_amdgcn_fence(order, scope) ((void)0)
+#define __builtin_amdgcn_sched_barrier(mask) ((void)0)
 
 // Cross-lane exchange within a wavefront.
 inline double __shfl_xor(double v, int mask, int width = 64) {
@@ -101,6 +103,18 @@ inline double __shfl(double v, int src, int width = 64) {
   s.bar->arrive_and_wait();
   return r;
 }
+
+// DPP lane exchange within 16-lane rows for the controls the kernels use: quad_perm
+// (ctrl < 0x100) and row_ror:n (0x121..0x12F).
+inline int emu_mov_dpp(int v, int ctrl) {
+  const int lane = threadIdx.x & 63;
+  int src = lane;
+  if (ctrl < 0x100) src = (lane & ~3) | ((ctrl >> (2 * (lane & 3))) & 3);
+  else if (ctrl >= 0x121 && ctrl <= 0x12F) src = (lane & ~15) | ((lane - (ctrl & 15)) & 15);
+  else std::abort();
+  return (int)(long long)__shfl((double)v, src);
+}
+#define __builtin_amdgcn_mov_dpp(v, ctrl, rm, bm, bc) emu_mov_dpp((v), (ctrl))
 
 // Scalar broadcast of lane l's value (l uniform across the wavefront).
 inline int emu_readlane_i32(int v, int l) { return (int)(long long)__shfl((double)v, l); }
