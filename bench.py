@@ -168,7 +168,23 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-kind", default="flip", help="kernel kind timed with HIP events: flip|gemm|qr|solve")
+    ap.add_argument("--traffic", default=None,
+                    help="profiles/ summary (JSON) holding the measured HBM bytes per launch of the profiled "
+                         "kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device once (the reference kernel for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 8}), flush=True)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs on single-GPU boxes; the chains of the "
@@ -239,6 +255,13 @@ def main():
     if prof:
         avg_ms, launches, work = prof
         out["roofline"] = roofline(args.profile_kind, avg_ms, launches, work, specs, args, chosen)
+        tpath = args.traffic or os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (args.profile_kind, args.config))
+        if os.path.exists(tpath):
+            with open(tpath) as f:
+                tr = json.load(f)
+            if tr.get("delay") in (None, args.delay):
+                out["roofline"]["traffic"] = tr["bytes_per_launch"]
+                out["roofline"]["traffic_source"] = os.path.relpath(tpath, ROOT)
     if not args.no_cpu:
         log("[bench] CPU baseline (bounded sample)")
         out["cpu_baseline"] = cpu_baseline(cfg)
@@ -246,31 +269,40 @@ def main():
 
 
 def roofline(kind, avg_ms, launches, work, specs, args, form):
-    """Algorithmic flops of the profiled kernel kind per launch over its mean duration.
+    """Roofline record of the profiled kernel kind: algorithmic work per launch over the
+    mean launch duration (HIP events on the engine stream).
 
-    flip: the delayed form's rank-k products are 4*a*N^2 flops per attempted flip
-    (both spins, a = acceptance), on the FP64 (or FP32) matrix cores; the rank-1
-    form streams G instead: 4*a*N^2*s bytes per attempt (s = element size).
-    gemm/qr/solve: the flops recorded by the engine per launch.
+    flip, delayed form: each block of k sites reads and writes both spins' G once
+    (G += P Z), 2 * 2 * N^2 * s bytes per block = 4 N^2 s / k bytes per attempted flip
+    (s = element size); with 4 a N^2 flops per attempt the arithmetic intensity is
+    a k / s flop/byte (~1.3 at a = 0.65, k = 16, FP64), far below the FP64 ridge point
+    (78.6 TFLOP/s / 8 TB/s ~ 10), so the bound is memory.  The flop rate is reported
+    beside it.
+    flip, rank-1 form: 4 a N^2 s bytes per attempt (G streamed per accepted flip).
+    gemm/qr/solve: the flops recorded by the engine per launch, against the FP64 peak.
     """
     st = form["stats"]
     acc = st["accepted"] / max(st["attempted"], 1)
     n = specs[0].n_sites
+    esz = 4 if args.precision == "fp32" else 8
     peak_mfma = PEAK["fp32"] if args.precision == "fp32" else PEAK["fp64"]
     if kind == "flip":
         flips_launch = work / max(launches, 1)
         if args.delay > 1:
+            byts = 4.0 * n * n * esz / args.delay * flips_launch
             flops = 4.0 * acc * n * n * flips_launch
-            achieved = flops / (avg_ms * 1e-3) / 1e12
-            return {"bound": "mfma", "kernel": "flip (delayed rank-k)", "achieved": achieved, "peak": peak_mfma,
-                    "unit": "TFLOP/s", "frac": achieved / peak_mfma, "traffic": None, "avg_ms": avg_ms,
-                    "launches": launches, "flops_per_launch": flops, "per_unit": "4*a*N^2 flops per attempt"}
-        esz = 4 if args.precision == "fp32" else 8
+            achieved = byts / (avg_ms * 1e-3) / 1e9
+            return {"bound": "hbm", "kernel": "k_flip_delayed (delayed rank-k, k=%d)" % args.delay,
+                    "achieved": achieved, "peak": PEAK["hbm"], "unit": "GB/s", "frac": achieved / PEAK["hbm"],
+                    "traffic": None, "avg_ms": avg_ms, "launches": launches, "bytes_per_launch": byts,
+                    "per_unit": "4*N^2*s/k bytes per attempted flip (G read+write once per block of k sites)",
+                    "flops_per_launch": flops, "tflops": flops / (avg_ms * 1e-3) / 1e12,
+                    "tflops_frac_of_mfma_peak": flops / (avg_ms * 1e-3) / 1e12 / peak_mfma}
         byts = 4.0 * acc * n * n * esz * flips_launch
         achieved = byts / (avg_ms * 1e-3) / 1e9
-        return {"bound": "hbm", "kernel": "flip (rank-1)", "achieved": achieved, "peak": PEAK["hbm"], "unit": "GB/s",
+        return {"bound": "hbm", "kernel": "k_flip_rank1", "achieved": achieved, "peak": PEAK["hbm"], "unit": "GB/s",
                 "frac": achieved / PEAK["hbm"], "traffic": None, "avg_ms": avg_ms, "launches": launches,
-                "bytes_per_launch": byts, "per_unit": "4*a*N^2*s bytes per attempt"}
+                "bytes_per_launch": byts, "per_unit": "4*a*N^2*s bytes per attempted flip"}
     flops = work / max(launches, 1)
     achieved = flops / (avg_ms * 1e-3) / 1e12
     peak = peak_mfma if kind == "gemm" else PEAK["fp64"]
