@@ -116,6 +116,13 @@ int lqmc_get_green(lqmc_ctx* ctx, int c, double* out, int which);
 /* Same, into device memory (e.g. a torch tensor's data_ptr()), on the given stream. */
 int lqmc_get_green_dev(lqmc_ctx* ctx, int c, double* dev_out, int which, void* stream);
 
+/* Mean over the measured sweeps of the equal-time spin correlation <S^z_i S^z_j>
+ * (S^z = n_up - n_dn, out N x N row-major), estimated from the end-of-sweep G of each
+ * measured sweep by Wick's theorem.  Extends the measurement of
+ * lqmc/lqmc.py:371-375 (which accumulates G only); the reference derives its local
+ * moment from the averaged G instead (lqmc/tools.py:87-109). */
+int lqmc_get_spin_corr(lqmc_ctx* ctx, int c, double* out);
+
 /* Current field of chain c into out[N*L] in the reference layout. */
 int lqmc_get_field(lqmc_ctx* ctx, int c, int8_t* out);
 

@@ -45,12 +45,13 @@ struct ChainDev {
   void* G;             // [2][N][N] in the form's precision
   double* Gacc;        // [2][N][N] sum of measured G
   double* Glast;       // [2][N][N] last measured G
+  double* Sacc;        // [N][N] sum over measured sweeps of <S^z_i S^z_j>, S^z = n_up - n_dn
   double* hop;         // [N][N] off-diagonal part of K
   double* uni;         // replay uniforms [n_replay][L][N]
   double* tr_ratio;    // [L][N] trace
   uint8_t* tr_acc;     // [L][N] trace
   double* obs;         // [obs_cap][3]
-  void* dws;           // delayed-update operands: [2 spins][P: Np x pld | Z: K4 x Np], form precision
+  void* dws;           // delayed-update workspace: Z tiles of both spins in register order
   int obs_cap;
   int pad1;
 };

@@ -84,7 +84,7 @@ struct Chain {
   int8_t* field = nullptr;
   double *ev = nullptr, *evi = nullptr;
   void* G = nullptr;
-  double *Gacc = nullptr, *Glast = nullptr, *Gst = nullptr, *hop = nullptr;
+  double *Gacc = nullptr, *Glast = nullptr, *Gst = nullptr, *hop = nullptr, *Sacc = nullptr;
   double *tr_ratio = nullptr, *obs = nullptr;
   uint8_t* tr_acc = nullptr;
   double *expk = nullptr, *expki = nullptr;
@@ -163,6 +163,7 @@ static void layout_chain(const lqmc_ctx* x, Chain& c, char* base, size_t& off) {
   c.G = g;
   carve(base, off, c.Gacc, 2 * n2);
   carve(base, off, c.Glast, 2 * n2);
+  carve(base, off, c.Sacc, n2);
   carve(base, off, c.Gst, 2 * n2);
   carve(base, off, c.hop, n2);
   carve(base, off, c.obs, 3 * (size_t)x->opt.obs_capacity);
@@ -698,7 +699,7 @@ static void refresh_chain_desc(lqmc_ctx* x, int ci) {
   d.e2p = std::exp(2.0 * c.lam); d.e2m = std::exp(-2.0 * c.lam);
   d.seed = c.seed;
   d.field = c.field; d.ev = c.ev; d.evi = c.evi; d.G = c.G;
-  d.Gacc = c.Gacc; d.Glast = c.Glast; d.hop = c.hop; d.uni = c.uni;
+  d.Gacc = c.Gacc; d.Glast = c.Glast; d.Sacc = c.Sacc; d.hop = c.hop; d.uni = c.uni;
   d.tr_ratio = c.tr_ratio; d.tr_acc = c.tr_acc; d.obs = c.obs; d.obs_cap = x->opt.obs_capacity;
   d.dws = c.dws;
   x->h_chains[ci] = d;
@@ -895,6 +896,20 @@ int lqmc_get_green_dev(lqmc_ctx* x, int ci, double* dev_out, int which, void* st
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : x->stream;
   HIPCHK(hipMemcpyAsync(dev_out, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, s));
   HIPCHK(hipStreamSynchronize(s));
+  return LQMC_OK;
+}
+
+int lqmc_get_spin_corr(lqmc_ctx* x, int ci, double* out) {
+  if (!x || ci < 0 || ci >= x->nchains || !out) return fail(LQMC_EINVAL, "lqmc_get_spin_corr: bad arguments");
+  HIPCHK(hipSetDevice(x->device));
+  Chain& c = x->ch[ci];
+  const size_t n2 = (size_t)c.N * c.N;
+  ChainState st;
+  HIPCHK(hipStreamSynchronize(x->stream));
+  HIPCHK(hipMemcpy(&st, x->d_state + ci, sizeof st, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(out, c.Sacc, 8 * n2, hipMemcpyDeviceToHost));
+  const double m = st.measured ? (double)st.measured : 1.0;
+  for (size_t e = 0; e < n2; ++e) out[e] /= m;
   return LQMC_OK;
 }
 

@@ -102,6 +102,19 @@ __device__ void slice_epilogue(const FlipArgs& a, const ChainDev& ch, int c, int
       ch.Gacc[e] += g;
       ch.Glast[e] = g;
     }
+    // <S^z_i S^z_j> by Wick's theorem for the sampled field, S^z = n_up - n_dn:
+    // m_i m_j + sum_s (delta_ij - G_s,ji) G_s,ij with m_i = G_dn,ii - G_up,ii
+    for (size_t e = threadIdx.x; e < n2; e += blockDim.x) {
+      const size_t i = e / N, j = e % N;
+      const double mi = (double)G[n2 + i * N + i] - (double)G[i * N + i];
+      const double mj = (double)G[n2 + j * N + j] - (double)G[j * N + j];
+      double v = mi * mj;
+      for (int sp = 0; sp < 2; ++sp) {
+        const double gij = (double)G[sp * n2 + i * N + j], gji = (double)G[sp * n2 + j * N + i];
+        v += ((i == j ? 1.0 : 0.0) - gji) * gij;
+      }
+      ch.Sacc[e] += v;
+    }
     double dens = 0.0, docc = 0.0, ek = 0.0;
     for (int i = threadIdx.x; i < N; i += blockDim.x) {
       const double nu = 1.0 - (double)G[(size_t)i * N + i];

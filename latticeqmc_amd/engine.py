@@ -122,6 +122,13 @@ class SweepEngine:
         nat.check(self.lib.lqmc_get_green(self.h, i, self._dp(out), int(which)), "lqmc_get_green")
         return out
 
+    def spin_correlation(self, i):
+        """Mean over the measured sweeps of <S^z_i S^z_j> (S^z = n_up - n_dn), (N, N)."""
+        n = self.chains[i].n_sites
+        out = np.empty((n, n), dtype=np.float64)
+        nat.check(self.lib.lqmc_get_spin_corr(self.h, i, self._dp(out)), "lqmc_get_spin_corr")
+        return out
+
     def field(self, i):
         c = self.chains[i]
         out = np.empty((c.n_sites, c.time_steps), dtype=np.int8)

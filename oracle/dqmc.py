@@ -261,3 +261,16 @@ def observables(g_up, g_dn, hop):
     for g in (g_up, g_dn):
         ekin += float(np.sum(hop * (np.eye(n) - g.T)))
     return dict(density=dens, docc=docc, ekin=ekin / n)
+
+
+def spin_correlation(g_up, g_dn):
+    """Equal-time <S^z_i S^z_j>, S^z = n_up - n_dn, for one field configuration by Wick's
+    theorem from G_ij = <c_i c_j^+>:  m_i m_j + sum_s (delta_ij - G_s,ji) G_s,ij  with
+    m_i = G_dn,ii - G_up,ii.  (The reference measures G only, lqmc.py:371-375; this is
+    the estimator behind its local-moment helper, tools.py:87-109, taken per sweep.)"""
+    n = g_up.shape[0]
+    m = np.diag(g_dn) - np.diag(g_up)
+    c = np.outer(m, m)
+    for g in (g_up, g_dn):
+        c += (np.eye(n) - g.T) * g
+    return c

@@ -81,3 +81,39 @@ def free_g(k, beta):
     """U = 0 equal-time G = (I + e^{-beta K})^-1 via the eigenbasis of K."""
     w, v = np.linalg.eigh(k)
     return (v / (1.0 + np.exp(-beta * w))) @ v.T
+
+
+def fock_one_body(c, a, modes):
+    """Fock-space operator sum_ij a_ij c+_{modes[i]} c_{modes[j]} from annihilators c."""
+    op = np.zeros_like(c[0])
+    for i, mi in enumerate(modes):
+        for j, mj in enumerate(modes):
+            if a[i, j] != 0.0:
+                op += a[i, j] * (c[mi].T @ c[mj])
+    return op
+
+
+def fock_product_expect(logs_up, logs_dn):
+    """Many-body expectations for ONE auxiliary-field configuration, by brute force in
+    Fock space.  The weight is B^ = prod_t e^{A^_t} (t in list order) for each spin, with
+    A^_t = sum_ij (A_t)_ij c+_i c_j the one-body operator of the N x N matrix A_t.
+    Returns (G_up, G_dn, S): G_s,ij = Tr[c_is c+_js B^]/Tr[B^] and
+    S_ij = Tr[S^z_i S^z_j B^]/Tr[B^], S^z = n_up - n_dn.  Checks the Wick estimators of a
+    non-symmetric G against their definition."""
+    n = logs_up[0].shape[0]
+    c = _annihilators(2 * n)
+    b = np.eye(len(c[0]))
+    for s, logs in enumerate((logs_up, logs_dn)):
+        modes = [s * n + i for i in range(n)]
+        for a in logs:
+            b = b @ expm(fock_one_body(c, a, modes))
+    z = np.trace(b)
+    g = np.zeros((2, n, n))
+    for s in range(2):
+        for i in range(n):
+            for j in range(n):
+                g[s, i, j] = np.trace(c[s * n + i] @ c[s * n + j].T @ b) / z
+    num = [c[m].T @ c[m] for m in range(2 * n)]
+    sz = [num[i] - num[n + i] for i in range(n)]
+    sc = np.array([[np.trace(sz[i] @ sz[j] @ b) / z for j in range(n)] for i in range(n)])
+    return g[0], g[1], sc

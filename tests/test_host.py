@@ -56,3 +56,21 @@ def test_native_load_declares_prototypes():
     from latticeqmc_amd import _native
     lib = _native.load()
     assert lib.lqmc_last_error() is not None
+
+
+def test_jackknife_and_structure_factor():
+    from latticeqmc_amd import tools
+    rs = np.random.RandomState(0)
+    a = 2.0 + rs.randn(4000)
+    b = 4.0 + rs.randn(4000)
+    est, err = tools.jackknife(lambda x, y: x / y, a, b, n_bins=40)
+    assert abs(est - 0.5) < 4 * err and 0.002 < err < 0.02
+    m, e = tools.jackknife(lambda x: x, a, n_bins=40)
+    bm, be = tools.binned_mean(a, n_bins=40)
+    assert abs(m - bm) < 1e-12 and abs(e - be) < 1e-12      # linear f: jackknife == binning
+    # Neel pattern on a 4x4 square: S(pi, pi) = N, S(0, 0) = 0
+    pos = np.array([(x, y) for x in range(4) for y in range(4)], dtype=float)
+    s = np.array([1.0 if (x + y) % 2 == 0 else -1.0 for x, y in pos])
+    c = np.outer(s, s)
+    assert abs(tools.structure_factor(c, pos, (np.pi, np.pi)) - 16.0) < 1e-12
+    assert abs(tools.structure_factor(c, pos, (0.0, 0.0))) < 1e-12

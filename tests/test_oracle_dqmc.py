@@ -121,3 +121,34 @@ def test_2x2_physics_matches_trotter_exact():
     me, se = binned_mean(e, 20)
     assert abs(md - ex["docc"]) < 4 * sd + 1e-4
     assert abs(me - ex["ekin"]) < 4 * se + 1e-4
+
+
+def test_spin_correlation_matches_fock_space():
+    """The Wick estimator of <S^z_i S^z_j> for one field configuration equals its Fock-space
+    definition Tr[S_i S_j B^]/Tr[B^] (B^ the many-body image of B(l)...B(l+1)); the same
+    brute force pins G = (I + B...B)^-1 and its transposition convention."""
+    from oracle import exact
+    k = model.square_k(2, 2)
+    beta, L, u = 1.5, 6, 4.0
+    dtau, lam, ek, eki = model.params(k, beta, L, u)
+    f = model.random_field(4, L, 3)
+    o = dqmc.DQMCOracle(k, beta, L, u, f, n_stab=3)
+    l = 2
+    order = [(l - j) % L for j in range(L)]      # B(l) B(l-1) ... B(l+1), B = V E_K
+    logs = []
+    for sigma in (+1, -1):
+        seq = []
+        for ll in order:
+            seq.append(np.diag(-sigma * lam * f[:, ll].astype(float)))
+            seq.append(-dtau * k)
+        logs.append(seq)
+    g_up, g_dn, s_exact = exact.fock_product_expect(logs[0], logs[1])
+    assert np.abs(g_up - o.exact_g(l, +1)).max() < 1e-10
+    assert np.abs(g_dn - o.exact_g(l, -1)).max() < 1e-10
+    assert np.abs(g_up - g_up.T).max() > 1e-3     # non-symmetric: transposition matters
+    s_wick = dqmc.spin_correlation(g_up, g_dn)
+    assert np.abs(s_wick - s_exact).max() < 1e-10
+    # a transposed contraction (a plausible slip) is caught
+    m = np.diag(g_dn) - np.diag(g_up)
+    wrong = np.outer(m, m) + sum((np.eye(4) - g) * g for g in (g_up, g_dn))
+    assert np.abs(wrong - s_exact).max() > 1e-6
