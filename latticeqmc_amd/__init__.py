@@ -11,10 +11,10 @@ from .hubbard import HubbardModel
 from .tools import check_params, filling, local_moment, local_gf, binned_mean
 from .engine import ChainSpec, SweepEngine, slice_params
 from .lqmc import LatticeQMC
-from .batched import BatchedDQMC, measure, measure_betas
+from .batched import BatchedDQMC, get_datadir, get_datapath, measure, measure_betas, measure_scan
 
 __all__ = [
     "Lattice", "Configuration", "HubbardModel", "LatticeQMC", "ChainSpec", "SweepEngine",
-    "BatchedDQMC", "measure", "measure_betas", "check_params", "filling", "local_moment",
+    "BatchedDQMC", "measure", "measure_betas", "measure_scan", "get_datadir", "get_datapath", "check_params", "filling", "local_moment",
     "local_gf", "binned_mean", "slice_params",
 ]
