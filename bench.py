@@ -202,10 +202,13 @@ def main():
     else:
         entry = autotune.lookup(specs, args.precision, path=cache)
     tuned = entry is not None
+    # untuned defaults per workload: the recompute interval keeps max|dG| <= 1e-6 with margin
+    # (24x24 U=6 beta=10 and the U<=8 batch need n_s=6; n_s=8 drifts to 1e-5 / 4e-6 there)
+    dflt = {"c4": (16, 8), "c2": (8, 8), "c3": (8, 6), "c5": (16, 6)}.get(args.config, (16, 6))
     if not args.delay:
-        args.delay = entry["delay"] if tuned else 16
+        args.delay = entry["delay"] if tuned else dflt[0]
     if not args.n_stab:
-        args.n_stab = entry["n_stab"] if tuned else 8
+        args.n_stab = entry["n_stab"] if tuned else dflt[1]
     log(f"[bench] chosen form: {args.precision} delay={args.delay} n_stab={args.n_stab} graph={not args.no_graph}")
     chosen = time_form(specs, args.steps, args.warmup, profile_kind=args.profile_kind, mode="dqmc",
                        precision=args.precision, delay=args.delay, n_stab=args.n_stab, graph=not args.no_graph)

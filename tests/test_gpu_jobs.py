@@ -1,5 +1,7 @@
 """Job layer on the GPU: measure_betas and the U x beta scan run their chains in one
-batched launch; at mu = U/2 on the bipartite square lattice the filling per spin is 1/2."""
+batched launch.  The interaction is decoupled in the particle-hole symmetric form
+U (n_up - 1/2)(n_dn - 1/2) while K carries -mu, so mu = 0 is half filling (filling 1/2 per
+spin on the bipartite square lattice); the reference's default mu = U/2 is not."""
 import os
 import sys
 
@@ -14,7 +16,7 @@ pytestmark = pytest.mark.gpu
 def test_measure_functions_on_gpu(tmp_path):
     print("[jobs]", flush=True)
     sys.stdout.flush()
-    m = HubbardModel(u=4.0)
+    m = HubbardModel(u=4.0, mu=0.0)
     m.build_square(2)
     tmp = str(tmp_path / "tmp_gf_series.npz")
     up, dn = measure_betas(m, [1.0, 2.0], 16, 20, 100, tmp_file=tmp, n_stab=8)
