@@ -107,10 +107,30 @@ def record_lattices():
     m = lqmc.HubbardModel(u=2, t=0.5, mu=0.25)
     m.build(3, 4, cycling=(0, 1))
     out["K_rect_3x4"] = m.ham_kinetic()
+    # other prefabs (lattice.py:365-421): open hexagonal (two-atom cell), cubic open and
+    # periodic, and a square lattice carrying two distance shells
+    from lqmc.lattice import Lattice
+    for name, make, shape, pbc in (("K_hex_2x2_open", Lattice.hexagonal, (2, 2), None),
+                                   ("K_hex_3x2_open", Lattice.hexagonal, (3, 2), None),
+                                   ("K_cubic_3x3x3_open", Lattice.cubic, (3, 3, 3), None),
+                                   ("K_cubic_3x3x3_pbc", Lattice.cubic, (3, 3, 3), (0, 1, 2)),
+                                   ("K_square_3x5_open_nn2", lambda: Lattice.square(neighbour_dist=2), (3, 5), None)):
+        latt = make()
+        latt.build(shape)
+        if pbc is not None:
+            latt.set_periodic_boundary(pbc)
+        m = lqmc.HubbardModel(u=4, t=1.0, mu=0.5)
+        m.lattice = latt
+        out[name] = m.ham_kinetic()
     return out
 
 
 def main():
+    if "--lattices-only" in sys.argv:
+        lat = record_lattices()
+        np.savez_compressed(os.path.join(HERE, "lattices.npz"), **lat)
+        print("lattices.npz", list(lat))
+        return
     cases = {
         "mode_r_2x2.npz": dict(size=2, u=4.0, beta=2.0, L=16, seed=0, n_sweeps=2),
         "mode_r_4x4.npz": dict(size=4, u=4.0, beta=2.0, L=16, seed=0, n_sweeps=2),

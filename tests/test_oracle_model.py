@@ -49,3 +49,36 @@ def test_params_closed_forms():
     assert np.abs(ek - (v * np.exp(-dtau * w)) @ v.T).max() < 1e-13
     assert np.abs(ek @ eki - np.eye(16)).max() < 1e-13
     assert model.params(k, 2.0, 16, 0.0)[1] == 0.0
+
+
+@pytest.mark.parametrize("name,make,shape,pbc", [
+    ("K_hex_2x2_open", "hexagonal", (2, 2), None),
+    ("K_hex_3x2_open", "hexagonal", (3, 2), None),
+    ("K_cubic_3x3x3_open", "cubic", (3, 3, 3), None),
+    ("K_cubic_3x3x3_pbc", "cubic", (3, 3, 3), (0, 1, 2)),
+    ("K_square_3x5_open_nn2", "square_nn2", (3, 5), None),
+])
+def test_other_lattices_match_reference(golden_dir, name, make, shape, pbc):
+    """Two-atom hexagonal cells, the cubic lattice (sites ordered x, z, y) and a square
+    lattice carrying two distance shells give the reference's K (lattice.py:365-421)."""
+    from latticeqmc_amd.lattice import Lattice
+    g = np.load(f"{golden_dir}/lattices.npz")
+    latt = Lattice.square(neighbour_dist=2) if make == "square_nn2" else getattr(Lattice, make)()
+    latt.build(shape)
+    if pbc is not None:
+        latt.set_periodic_boundary(pbc)
+    m = HubbardModel(u=4.0, t=1.0, mu=0.5)
+    m.lattice = latt
+    assert np.array_equal(m.ham_kinetic(), g[name])
+
+
+def test_skewed_cell_periodic_boundary_raises():
+    """The reference raises on periodic boundaries of the hexagonal lattice (numpy >= 2.2);
+    so does this one, with a clear message."""
+    from latticeqmc_amd.lattice import Lattice
+    latt = Lattice.hexagonal()
+    latt.build((3, 3))
+    with pytest.raises(ValueError, match="orthogonal"):
+        latt.set_periodic_boundary(0)
+    # distance shells of the honeycomb: 1, sqrt(3), 2
+    assert np.allclose(Lattice.hexagonal(neighbour_dist=3).distances, [1.0, np.sqrt(3), 2.0], atol=1e-5)
