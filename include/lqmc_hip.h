@@ -39,7 +39,7 @@ enum {
   LQMC_ENUMERIC = -5,  /* device flagged NaN/Inf, zero pivot or replay underrun */
 };
 
-enum { LQMC_MODE_DQMC = 0, LQMC_MODE_REFERENCE = 1 };
+enum { LQMC_MODE_DQMC = 0, LQMC_MODE_REFERENCE = 1, LQMC_MODE_DET = 2 };
 enum { LQMC_FP64 = 0, LQMC_FP32 = 1 };
 enum { LQMC_RNG_REPLAY = 0, LQMC_RNG_PHILOX = 1 };
 
@@ -52,7 +52,9 @@ enum {
 };
 
 typedef struct {
-  int32_t mode;        /* LQMC_MODE_DQMC (textbook, stabilised) or LQMC_MODE_REFERENCE */
+  int32_t mode;        /* LQMC_MODE_DQMC (textbook, stabilised), LQMC_MODE_REFERENCE, or
+                          LQMC_MODE_DET: the reference's brute-force determinant sweep
+                          (lqmc/lqmc.py:236-299), FP64 rank-1 only, N <= 32 */
   int32_t precision;   /* LQMC_FP64 or LQMC_FP32: storage of G, wraps and flip updates */
   int32_t delay;       /* 1: rank-1 update per accepted flip; k>1: submatrix blocks of k sites */
   int32_t n_stab;      /* slices between FP64 QR/UDV recomputes (Mode D) */

@@ -9,7 +9,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblqmc_hip.so")
 
-MODE_DQMC, MODE_REFERENCE = 0, 1
+MODE_DQMC, MODE_REFERENCE, MODE_DET = 0, 1, 2
 FP64, FP32 = 0, 1
 RNG_REPLAY, RNG_PHILOX = 0, 1
 

@@ -59,7 +59,7 @@ def measure(model, beta, time_steps, warmup, sweeps, cores=None, det_mode=False,
     the reference's ParallelProcessManager does (multiprocessing.py:260-267).
     """
     if det_mode:
-        raise NotImplementedError("det_mode is not provided by the GPU engine")
+        engine_kw.update(mode="det", precision="fp64", delay=1)
     if cores is not None and cores == 1:
         from .lqmc import LatticeQMC
         solver = LatticeQMC(model, beta, time_steps, warmup, sweeps, **engine_kw)
@@ -113,7 +113,7 @@ def measure_betas(model, betas, time_steps, warmup=500, sweeps=5000, cores=-1, c
     removed when all betas are done.  ``cores`` is accepted for signature compatibility.
     """
     if det_mode:
-        raise NotImplementedError("det_mode is not provided by the GPU engine")
+        engine_kw.update(mode="det", precision="fp64", delay=1)
     runner = runner or gpu_runner
     betas = np.asarray(betas, dtype=np.float64)
     n = model.n_sites

@@ -47,6 +47,7 @@ struct ChainDev {
   double* Glast;       // [2][N][N] last measured G
   double* Sacc;        // [N][N] sum over measured sweeps of <S^z_i S^z_j>, S^z = n_up - n_dn
   double* hop;         // [N][N] off-diagonal part of K
+  const double* expk;  // [N][N] E_K = expm(-dtau K)
   double* uni;         // replay uniforms [n_replay][L][N]
   double* tr_ratio;    // [L][N] trace
   uint8_t* tr_acc;     // [L][N] trace

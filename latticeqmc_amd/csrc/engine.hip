@@ -41,6 +41,9 @@ size_t flip_delayed_shmem(int nmax, int k, int threads, int fp32);
 size_t flip_delayed_workspace(int n, int k, int fp32);
 int flip_delayed_threads_limit(int k);
 size_t qr_blocked_shmem(int nmax);
+size_t det_sweep_shmem(int nmax, int lmax);
+void launch_det_sweep(const ChainDev* chains, ChainState* st, int nchains, int measure, int rng, int trace,
+                      size_t shmem, hipStream_t s);
 }  // namespace lqmc
 
 using namespace lqmc;
@@ -61,7 +64,7 @@ static int fail(int code, const std::string& msg) {
 
 namespace {
 
-enum OpKind { OP_GEMM = 0, OP_QR, OP_SOLVE, OP_FINISH, OP_FLIP, OP_EXPV, OP_IDENT, OP_TRSM };
+enum OpKind { OP_GEMM = 0, OP_QR, OP_SOLVE, OP_FINISH, OP_FLIP, OP_EXPV, OP_IDENT, OP_TRSM, OP_DET };
 enum ProfKind { PK_FLIP = 0, PK_GEMM = 1, PK_QR = 2, PK_SOLVE = 3 };
 
 struct Op {
@@ -132,6 +135,7 @@ struct lqmc_ctx {
   int nmax = 0, lmax = 0;
   int flip_threads = 256;
   size_t flip_shmem = 0;
+  size_t det_shmem = 0;
   // graphs
   bool use_graph = false;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -153,7 +157,7 @@ static void carve(char* base, size_t& off, T*& p, size_t count) {
 static void layout_chain(const lqmc_ctx* x, Chain& c, char* base, size_t& off) {
   const size_t N = c.N, L = c.L, n2 = N * N;
   const bool fp32 = x->opt.precision == LQMC_FP32;
-  const bool ref = x->opt.mode == LQMC_MODE_REFERENCE;
+  const bool dq = x->opt.mode == LQMC_MODE_DQMC;
   const size_t esz = fp32 ? 4 : 8;
   carve(base, off, c.field, N * L);
   carve(base, off, c.ev, 2 * N * L);
@@ -191,7 +195,7 @@ static void layout_chain(const lqmc_ctx* x, Chain& c, char* base, size_t& off) {
     char* tw = nullptr;
     carve(base, off, tw, n2 * esz);
     c.Tw[s] = tw;
-    if (!ref) {
+    if (dq) {
       carve(base, off, c.LU[s], (size_t)c.nb * n2);
       carve(base, off, c.LD[s], (size_t)c.nb * N);
       carve(base, off, c.LV[s], (size_t)c.nb * n2);
@@ -316,7 +320,12 @@ static void build_program(lqmc_ctx* x) {
     return static_cast<char*>(c.G) + (size_t)s * c.N * c.N * (fp32 ? 4 : 8);
   };
 
-  if (ref) {
+  if (x->opt.mode == LQMC_MODE_DET) {
+    // ---- determinant mode: one brute-force sweep kernel per sweep ----
+    double w = 0;
+    for (auto& c : x->ch) w += (double)c.N * c.L;
+    x->sweep_ops.push_back(Op{OP_DET, 0, 0, nc, 0, 0, 0, w});
+  } else if (ref) {
     // ---- Mode R sweep: P = prod_{l=L-1..0} E_K V(l), G = inv(I + P) ----
     std::vector<int> cur(nc * 2, 0);  // 0 -> Wa, 1 -> Wb holds the running product
     for (int m = 0; m < x->lmax; ++m) {
@@ -553,7 +562,7 @@ static void run_ops(lqmc_ctx* x, const std::vector<Op>& ops, int measure, hipStr
   const bool fp32 = x->opt.precision == LQMC_FP32;
   for (const Op& op : ops) {
     int pk = -1;
-    if (op.kind == OP_FLIP) pk = PK_FLIP;
+    if (op.kind == OP_FLIP || op.kind == OP_DET) pk = PK_FLIP;
     else if (op.kind == OP_GEMM) pk = PK_GEMM;
     else if (op.kind == OP_QR) pk = PK_QR;
     else if (op.kind == OP_SOLVE || op.kind == OP_TRSM) pk = PK_SOLVE;
@@ -573,6 +582,9 @@ static void run_ops(lqmc_ctx* x, const std::vector<Op>& ops, int measure, hipStr
       case OP_FINISH: launch_finish(op.fp32, reinterpret_cast<const FinishTask*>(d), op.count, op.nmax, x->d_state, s); break;
       case OP_EXPV: launch_expv(x->d_chains, x->nchains, s); break;
       case OP_IDENT: launch_identity(x->d_eye_ptrs, x->d_eye_ns, x->nchains, s); break;
+      case OP_DET:
+        launch_det_sweep(x->d_chains, x->d_state, x->nchains, measure, x->opt.rng, x->opt.trace, x->det_shmem, s);
+        break;
       case OP_FLIP: {
         FlipArgs a{x->d_chains, x->d_state, op.t, measure, x->opt.rng, x->opt.trace, x->opt.delay, 0};
         launch_flip(fp32, x->opt.delay > 1, a, x->nchains, x->flip_threads, x->flip_shmem, s);
@@ -598,7 +610,10 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
   if (!out || n_chains <= 0 || !n_sites || !time_steps || !opt) return fail(LQMC_EINVAL, "lqmc_create: bad arguments");
   *out = nullptr;
   lqmc_options o = *opt;
-  if (o.mode != LQMC_MODE_DQMC && o.mode != LQMC_MODE_REFERENCE) return fail(LQMC_EINVAL, "unknown mode");
+  if (o.mode != LQMC_MODE_DQMC && o.mode != LQMC_MODE_REFERENCE && o.mode != LQMC_MODE_DET)
+    return fail(LQMC_EINVAL, "unknown mode");
+  if (o.mode == LQMC_MODE_DET && (o.precision != LQMC_FP64 || o.delay != 1))
+    return fail(LQMC_EINVAL, "determinant mode runs the FP64 rank-1 sweep only");
   if (o.precision != LQMC_FP64 && o.precision != LQMC_FP32) return fail(LQMC_EINVAL, "unknown precision");
   if (o.rng != LQMC_RNG_REPLAY && o.rng != LQMC_RNG_PHILOX) return fail(LQMC_EINVAL, "unknown rng");
   if (o.delay < 1 || o.delay > 64) return fail(LQMC_EINVAL, "delay must be in 1..64");
@@ -621,6 +636,10 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
     if (n_sites[c] < 1 || n_sites[c] > 1024 || time_steps[c] < 1) {
       delete x;
       return fail(LQMC_EINVAL, "chain sizes must satisfy 1 <= N <= 1024, L >= 1");
+    }
+    if (o.mode == LQMC_MODE_DET && n_sites[c] > 32) {
+      delete x;
+      return fail(LQMC_EINVAL, "determinant mode supports at most 32 sites per chain");
     }
     x->ch[c].N = n_sites[c];
     x->ch[c].L = time_steps[c];
@@ -650,6 +669,13 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
     snprintf(buf, sizeof buf, "flip kernel needs %zu B of LDS (> %d); lower the delay", x->flip_shmem, maxsh);
     delete x;
     return fail(LQMC_EINVAL, buf);
+  }
+  if (o.mode == LQMC_MODE_DET) {
+    x->det_shmem = det_sweep_shmem(x->nmax, x->lmax);
+    if (x->det_shmem > (size_t)maxsh) {
+      delete x;
+      return fail(LQMC_EINVAL, "determinant sweep needs more LDS than 64 KB; reduce time_steps");
+    }
   }
   // The blocked QR keeps one column panel of the largest chain in LDS.
   if (o.mode == LQMC_MODE_DQMC && qr_blocked_shmem(x->nmax) > (size_t)maxsh) {
@@ -700,6 +726,7 @@ static void refresh_chain_desc(lqmc_ctx* x, int ci) {
   d.seed = c.seed;
   d.field = c.field; d.ev = c.ev; d.evi = c.evi; d.G = c.G;
   d.Gacc = c.Gacc; d.Glast = c.Glast; d.Sacc = c.Sacc; d.hop = c.hop; d.uni = c.uni;
+  d.expk = c.expk;
   d.tr_ratio = c.tr_ratio; d.tr_acc = c.tr_acc; d.obs = c.obs; d.obs_cap = x->opt.obs_capacity;
   d.dws = c.dws;
   x->h_chains[ci] = d;

@@ -73,29 +73,14 @@ __device__ inline double wg_sum(double v, double* red) {
   return r;
 }
 
+// Measurement of one sweep from the equal-time G of both spins (lqmc.py:371-372 plus the
+// per-sweep estimators): accumulate G, keep it as the last measured G, accumulate
+// <S^z_i S^z_j>, and append density, double occupancy and kinetic energy per site to the
+// chain's observable ring.  All threads of the workgroup take part (reductions).
 template <typename T>
-__device__ void slice_epilogue(const FlipArgs& a, const ChainDev& ch, int c, int l, const T* G,
-                               unsigned long long nacc, bool underrun, bool bad, double* red) {
+__device__ void measure_equal_time(const ChainDev& ch, ChainState* st, const T* G, double* red) {
   const int N = ch.N;
-  const int NL = N * ch.L;
-  __syncthreads();
-  for (int i = threadIdx.x; i < N; i += blockDim.x) {
-    const int s = ch.field[(size_t)l * N + i];
-    const double e1m = ch.e1m, e1p = ch.e1p;  // values, so the conditionals select values
-    const double up = s > 0 ? e1m : e1p;
-    const double dn = s > 0 ? e1p : e1m;
-    ch.ev[(size_t)l * N + i] = up;
-    ch.ev[(size_t)NL + (size_t)l * N + i] = dn;
-    ch.evi[(size_t)l * N + i] = 1.0 / up;
-    ch.evi[(size_t)NL + (size_t)l * N + i] = 1.0 / dn;
-  }
-  if (threadIdx.x == 0) {
-    atomicAdd(&a.st[c].attempted, (unsigned long long)N);
-    atomicAdd(&a.st[c].accepted, nacc);
-    if (underrun) atomicOr(&a.st[c].flags, 4u);
-    if (bad) atomicOr(&a.st[c].flags, 1u);
-  }
-  if (l == 0 && a.measure) {
+  {
     const size_t n2 = (size_t)N * N;
     for (size_t e = threadIdx.x; e < 2 * n2; e += blockDim.x) {
       const double g = (double)G[e];
@@ -131,14 +116,39 @@ __device__ void slice_epilogue(const FlipArgs& a, const ChainDev& ch, int c, int
     docc = wg_sum(docc, red);
     ek = wg_sum(ek, red);
     if (threadIdx.x == 0) {
-      const unsigned long long k = a.st[c].measured;
+      const unsigned long long k = st->measured;
       double* o = ch.obs + (size_t)(k % (unsigned long long)ch.obs_cap) * 3;
       o[0] = dens / N;
       o[1] = docc / N;
       o[2] = ek / N;
-      a.st[c].measured = k + 1;
+      st->measured = k + 1;
     }
   }
+}
+
+template <typename T>
+__device__ void slice_epilogue(const FlipArgs& a, const ChainDev& ch, int c, int l, const T* G,
+                               unsigned long long nacc, bool underrun, bool bad, double* red) {
+  const int N = ch.N;
+  const int NL = N * ch.L;
+  __syncthreads();
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const int s = ch.field[(size_t)l * N + i];
+    const double e1m = ch.e1m, e1p = ch.e1p;  // values, so the conditionals select values
+    const double up = s > 0 ? e1m : e1p;
+    const double dn = s > 0 ? e1p : e1m;
+    ch.ev[(size_t)l * N + i] = up;
+    ch.ev[(size_t)NL + (size_t)l * N + i] = dn;
+    ch.evi[(size_t)l * N + i] = 1.0 / up;
+    ch.evi[(size_t)NL + (size_t)l * N + i] = 1.0 / dn;
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&a.st[c].attempted, (unsigned long long)N);
+    atomicAdd(&a.st[c].accepted, nacc);
+    if (underrun) atomicOr(&a.st[c].flags, 4u);
+    if (bad) atomicOr(&a.st[c].flags, 1u);
+  }
+  if (l == 0 && a.measure) measure_equal_time<T>(ch, &a.st[c], G, red);
   __syncthreads();
   if (l == 0 && threadIdx.x == 0) a.st[c].sweep += 1ull;
 }
@@ -563,6 +573,216 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
     __syncthreads();
   }
   slice_epilogue<T>(a, ch, c, l, G, nacc, underrun, bad, red);
+}
+
+// ---- determinant mode ------------------------------------------------------------------------
+// The reference's brute-force sweep (lqmc.py:236-299), one workgroup per chain, for lattices
+// of at most DET_NMAX sites (a cross-check of the fast sweeps on tiny lattices):
+//   M_s(l0) = I + prod over l in (l0-1, ..., 0, L-1, ..., l0) of E_K V_s(l)   (lqmc.py:156-185),
+//   for l = L-1 .. 0, i = 0 .. N-1: flip s_il, new = det M_up(l) det M_dn(l),
+//   accept iff u <= new / old (then old = new), else flip back;
+//   measurement: G_s = M_s(0)^-1 after the sweep.
+// The products and both LU factorisations (partial pivoting, first maximal |pivot| like
+// LAPACK's getrf) stay in LDS; wavefront s factors spin s.  The starting determinant is
+// recomputed from the field at the start of every sweep (the reference carries it from the
+// previous accepted flip; both are det M(0) up to rounding).
+constexpr int DET_NMAX = 32;
+
+struct DetArgs {
+  const ChainDev* chains;
+  ChainState* st;
+  int measure, rng, trace, pad;
+};
+
+// P_s <- prod over l in order(l0) of E_K V_s(l), both spins, all threads.  Buffer b (N x N) of
+// the ping-pong set starts at pp0 + b N^2; spin s uses buffers s and 2 + s.  Returns the
+// index (0 or 1) of the half holding the result.
+__device__ int det_product(const ChainDev& ch, const int8_t* fld, const double* ek, double* pp0, int l0) {
+  const int N = ch.N, L = ch.L;
+  const size_t n2 = (size_t)N * N;
+  for (size_t e = threadIdx.x; e < 2 * n2; e += blockDim.x) {
+    const int sp = (int)(e / n2);
+    const size_t r = (e % n2) / N, cc = (e % n2) % N;
+    pp0[sp * n2 + r * N + cc] = r == cc ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  int cur = 0;
+  const double e1m = ch.e1m, e1p = ch.e1p;
+  for (int q = 0; q < L; ++q) {
+    const int l = ((l0 - 1 - q) % L + L) % L;  // l0-1, ..., 0, L-1, ..., l0
+    for (size_t e = threadIdx.x; e < 2 * n2; e += blockDim.x) {
+      const int sp = (int)(e / n2);
+      const int r = (int)((e % n2) / N), cc = (int)((e % n2) % N);
+      const double* P = pp0 + (size_t)(cur * 2 + sp) * n2;
+      double acc = 0.0;
+      for (int k = 0; k < N; ++k) acc += P[r * N + k] * ek[k * N + cc];
+      const int sv = fld[l * N + cc];
+      // V_s(l) = exp(-s lambda s_l,c): sigma * s = +1 -> e^-lambda
+      const double v = (sp == 0) == (sv > 0) ? e1m : e1p;
+      pp0[(size_t)((1 - cur) * 2 + sp) * n2 + r * N + cc] = acc * v;
+    }
+    __syncthreads();
+    cur = 1 - cur;
+  }
+  return cur;
+}
+
+// In-place LU with partial pivoting of the N x N matrix A (wavefront-wide, lanes over rows);
+// returns det(A), records the row permutation in piv.
+__device__ double det_lu_wave(double* A, int N, int* piv) {
+  const int lane = threadIdx.x & 63;
+  double det = 1.0;
+  for (int k = 0; k < N; ++k) {
+    // |A[r][k]| over rows r >= k, NaN counted as +inf so that the order is total and every
+    // lane ends the butterfly on the same row; lanes outside [k, N) offer -1
+    double best = -1.0;
+    if (lane >= k && lane < N) {
+      const double v = fabs(A[lane * N + k]);
+      best = v == v ? v : INFINITY;
+    }
+    int bi = lane;
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ob = __shfl_xor(best, off, 64);
+      const int oi = __shfl_xor(bi, off, 64);
+      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+    }
+    const int p = min(max(__builtin_amdgcn_readlane(bi, 0), k), N - 1);
+    if (lane == 0) piv[k] = p;
+    if (p != k) {
+      for (int cc = lane; cc < N; cc += 64) {
+        const double t = A[k * N + cc];
+        A[k * N + cc] = A[p * N + cc];
+        A[p * N + cc] = t;
+      }
+      det = -det;
+    }
+    wave_sync();
+    const double d = A[k * N + k];
+    det *= d;
+    if (lane > k && lane < N && d != 0.0) {
+      const double lr = A[lane * N + k] / d;
+      A[lane * N + k] = lr;
+      for (int cc = k + 1; cc < N; ++cc) A[lane * N + cc] -= lr * A[k * N + cc];
+    }
+    wave_sync();
+  }
+  return det;
+}
+
+__global__ __launch_bounds__(256) void k_det_sweep(DetArgs a) {
+  const int c = blockIdx.x;
+  const ChainDev ch = a.chains[c];
+  const int N = ch.N, L = ch.L;
+  const size_t n2 = (size_t)N * N;
+  LQMC_DYNAMIC_LDS(double, sm);
+  double* ek = sm;                       // [N][N]
+  double* pp0 = ek + n2;                 // [4][N][N] ping-pong products / LU / inverse
+  double* red = ek + 5 * n2;             // [blockDim]
+  double* dets = red + blockDim.x;       // [2]
+  int* piv = reinterpret_cast<int*>(dets + 2);  // [2][DET_NMAX]
+  int8_t* fld = reinterpret_cast<int8_t*>(piv + 2 * DET_NMAX);  // [L][N]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (size_t e = threadIdx.x; e < n2; e += blockDim.x) ek[e] = ch.expk[e];
+  for (int e = threadIdx.x; e < N * L; e += blockDim.x) fld[e] = ch.field[e];
+  __syncthreads();
+
+  // determinants of I + P for both spins (wave s, spin s) -> dets[s]
+  auto both_dets = [&](int l0) {
+    const int cur = det_product(ch, fld, ek, pp0, l0);
+    if (wave < 2) {
+      double* A = pp0 + (size_t)(cur * 2 + wave) * n2;
+      for (int r = lane; r < N; r += 64) A[r * N + r] += 1.0;
+      wave_sync();
+      const double d = det_lu_wave(A, N, piv + wave * DET_NMAX);
+      if (lane == 0) dets[wave] = d;
+    }
+    __syncthreads();
+    return cur;
+  };
+
+  const unsigned long long sweep = a.st[c].sweep;
+  bool underrun = false, bad = false;
+  unsigned long long nacc = 0;
+  // det M(l) does not depend on l (det(I + AB) = det(I + BA) for the cyclic shifts), so the
+  // reference's running determinant equals det M(0) of the current field at every sweep start
+  both_dets(0);
+  double old_det = dets[0] * dets[1];
+  for (int t = 0; t < L; ++t) {
+    const int l = L - 1 - t;
+    for (int i = 0; i < N; ++i) {
+      if (threadIdx.x == 0) fld[l * N + i] = (int8_t)(-fld[l * N + i]);
+      __syncthreads();
+      both_dets(l);
+      const double new_det = dets[0] * dets[1];
+      const double ratio = new_det / old_det;
+      const double u = draw_uniform(ch, &a.st[c], c, a.rng, sweep, t, i, underrun);
+      const bool acc = u <= ratio;
+      if (!isfinite(ratio)) bad = true;
+      if (a.trace && threadIdx.x == 0) {
+        ch.tr_ratio[(size_t)t * N + i] = ratio;
+        ch.tr_acc[(size_t)t * N + i] = acc ? 1 : 0;
+      }
+      if (acc) {
+        old_det = new_det;
+        ++nacc;
+      } else if (threadIdx.x == 0) {
+        fld[l * N + i] = (int8_t)(-fld[l * N + i]);
+      }
+      __syncthreads();
+    }
+  }
+  for (int e = threadIdx.x; e < N * L; e += blockDim.x) ch.field[e] = fld[e];
+  if (threadIdx.x == 0) {
+    atomicAdd(&a.st[c].attempted, (unsigned long long)N * L);
+    atomicAdd(&a.st[c].accepted, nacc);
+    if (underrun) atomicOr(&a.st[c].flags, 4u);
+    if (bad) atomicOr(&a.st[c].flags, 1u);
+  }
+  if (a.measure) {
+    // G_s = (I + P_s(0))^-1 from the LU: lane j of wave s solves column j
+    const int cur = both_dets(0);
+    double* G = static_cast<double*>(ch.G);
+    if (wave < 2) {
+      const double* A = pp0 + (size_t)(cur * 2 + wave) * n2;
+      const int* pv = piv + wave * DET_NMAX;
+      double* X = pp0 + (size_t)((1 - cur) * 2 + wave) * n2;  // scratch [N][N], X[r][j]
+      const int j = lane;
+      if (j < N) {
+        for (int r = 0; r < N; ++r) X[r * N + j] = r == j ? 1.0 : 0.0;
+        for (int k = 0; k < N; ++k)  // the row swaps of the factorisation
+          if (pv[k] != k) {
+            const double t0 = X[k * N + j];
+            X[k * N + j] = X[pv[k] * N + j];
+            X[pv[k] * N + j] = t0;
+          }
+        for (int r = 1; r < N; ++r) {  // unit lower
+          double v = X[r * N + j];
+          for (int k = 0; k < r; ++k) v -= A[r * N + k] * X[k * N + j];
+          X[r * N + j] = v;
+        }
+        for (int r = N - 1; r >= 0; --r) {  // upper
+          double v = X[r * N + j];
+          for (int k = r + 1; k < N; ++k) v -= A[r * N + k] * X[k * N + j];
+          X[r * N + j] = v / A[r * N + r];
+        }
+        for (int r = 0; r < N; ++r) G[(size_t)wave * n2 + (size_t)r * N + j] = X[r * N + j];
+      }
+    }
+    __syncthreads();
+    measure_equal_time<double>(ch, &a.st[c], G, red);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) a.st[c].sweep += 1ull;
+}
+
+size_t det_sweep_shmem(int nmax, int lmax) {
+  return sizeof(double) * (5 * (size_t)nmax * nmax + 256 + 2) + sizeof(int) * 2 * DET_NMAX + (size_t)nmax * lmax;
+}
+
+void launch_det_sweep(const ChainDev* chains, ChainState* st, int nchains, int measure, int rng, int trace,
+                      size_t shmem, hipStream_t s) {
+  DetArgs a{chains, st, measure, rng, trace, 0};
+  hipLaunchKernelGGL(k_det_sweep, dim3(nchains), dim3(256), shmem, s, a);
 }
 
 int flip_delayed_threads_limit(int k) { return flip_delayed_max_threads(k); }

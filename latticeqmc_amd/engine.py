@@ -13,7 +13,7 @@ from scipy.linalg import expm
 
 from . import _native as nat
 
-_MODES = {"dqmc": nat.MODE_DQMC, "reference": nat.MODE_REFERENCE}
+_MODES = {"dqmc": nat.MODE_DQMC, "reference": nat.MODE_REFERENCE, "det": nat.MODE_DET}
 _PREC = {"fp64": nat.FP64, "fp32": nat.FP32}
 _RNG = {"replay": nat.RNG_REPLAY, "philox": nat.RNG_PHILOX}
 
@@ -56,7 +56,8 @@ class ChainSpec:
 class SweepEngine:
     """Batched DQMC sweeps of independent chains on one GPU.
 
-    mode      "dqmc" (textbook, QR/UDV-stabilised) or "reference" (literal
+    mode      "dqmc" (textbook, QR/UDV-stabilised), "det" (the reference's brute-force
+              determinant sweep, lqmc.py:236-299, N <= 32) or "reference" (literal
               lqmc/lqmc.py:301-347 math, FP64 rank-1, unstabilised)
     precision "fp64" or "fp32" (storage of G, wraps and flip updates; the
               stabilisation always runs in FP64)

@@ -17,8 +17,10 @@ Extra keyword-only options select the GPU form:
   reference's order, one per attempted flip, so a seeded run consumes the same
   stream as the reference; rng="philox" draws them on the device.
 
-``det_mode=True`` (the reference's brute-force determinant sweep,
-lqmc.py:236-299) is not provided by the GPU engine and raises.
+``det_mode=True`` runs the reference's brute-force determinant sweep
+(lqmc.py:236-299) on the device (lattices of at most 32 sites): every attempted
+flip recomputes det(I + prod E_K V) of both spins, and the measurement is the
+inverse of M(0) after each sweep.
 """
 import logging
 import time
@@ -36,8 +38,6 @@ class LatticeQMC:
     def __init__(self, model, beta, time_steps, warmup=300, sweeps=2000, det_mode=False, log_lvl=None,
                  *, mode="dqmc", precision="fp64", delay=1, n_stab=10, rng="numpy", seed=None,
                  device=0, graph=False, console=True):
-        if det_mode:
-            raise NotImplementedError("det_mode (brute-force determinant sweep) is not provided by the GPU engine")
         if log_lvl is not None:
             log.setLevel(log_lvl)
         self.model = model
@@ -58,6 +58,8 @@ class LatticeQMC:
         self.exp_k = None
         self.exp_k_inv = None
         self.mode, self.precision, self.delay, self.n_stab = mode, precision, delay, n_stab
+        if det_mode:  # the reference's determinant sweep (lqmc.py:236-299) on the device
+            self.mode, self.precision, self.delay = "det", "fp64", 1
         self.rng = rng
         self.seed = seed if seed is not None else int(np.random.randint(0, 2**31 - 1)) if rng == "philox" else 0
         self.device = device
@@ -149,6 +151,24 @@ class LatticeQMC:
         n = eng.stats(0)["measured"]
         total = eng.green(0, which=1) * n - g_before
         return total / max(self.meas_sweeps, 1)
+
+    # -- determinant sweep entry points (lqmc.py:261-299) ----------------------------------------
+    def _use_det(self):
+        """Switch this solver to the device determinant sweep, keeping the field."""
+        if self.mode != "det":
+            self._pull_config()
+            if self._engine is not None:
+                self._engine.close()
+            self._engine = None
+            self.mode, self.precision, self.delay = "det", "fp64", 1
+
+    def warmup_loop_det(self):
+        self._use_det()
+        self.warmup_loop()
+
+    def measure_loop_det(self):
+        self._use_det()
+        return self.measure_loop()
 
     def run_lqmc(self):
         self.warmup_loop()

@@ -121,8 +121,9 @@ def sweep_faithful(field, expk, lam, uniforms, max_slices=None):
     return q
 
 
-def det_sweep(field, expk, lam, uniforms, old_det):
-    """Brute-force determinant sweep (lqmc.py:236-259). Returns (old_det, accepts)."""
+def det_sweep(field, expk, lam, uniforms, old_det, ratios=None):
+    """Brute-force determinant sweep (lqmc.py:236-259). Returns (old_det, accepts); the
+    acceptance ratios are written into ``ratios`` (length N L) when it is given."""
     n, L = field.shape
     accs = np.zeros(n * L, dtype=bool)
     q = 0
@@ -134,6 +135,8 @@ def det_sweep(field, expk, lam, uniforms, old_det):
             ratio = new_det / old_det
             acc = uniforms[q] <= ratio
             accs[q] = acc
+            if ratios is not None:
+                ratios[q] = ratio
             q += 1
             if acc:
                 old_det = new_det

@@ -45,6 +45,7 @@ int main(int argc, char** argv) {
     }
     d.Gacc = dalloc<double>(2 * n2);
     d.Glast = dalloc<double>(2 * n2);
+    d.Sacc = dalloc<double>(n2);
     d.hop = dalloc<double>(n2);
     d.uni = nullptr;
     d.tr_ratio = dalloc<double>((size_t)N * L);
@@ -93,7 +94,7 @@ int main(int argc, char** argv) {
       double* g = dalloc<double>(2 * n2);
       for (size_t e = 0; e < 2 * n2; ++e) g[e] = gscale * U(rng2) / N + ((e % n2) / N == (e % n2) % N ? 0.5 : 0.0);
       d.G = g;
-      d.Gacc = dalloc<double>(2 * n2); d.Glast = dalloc<double>(2 * n2);
+      d.Gacc = dalloc<double>(2 * n2); d.Glast = dalloc<double>(2 * n2); d.Sacc = dalloc<double>(n2);
       d.ev = dalloc<double>(2 * (size_t)N * L); d.evi = dalloc<double>(2 * (size_t)N * L);
       d.tr_ratio = dalloc<double>((size_t)N * L); d.tr_acc = dalloc<uint8_t>((size_t)N * L);
       d.obs = dalloc<double>(3 * 8);

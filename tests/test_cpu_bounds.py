@@ -106,3 +106,34 @@ def test_blocked_qr_race_free(qr_harness_tsan, args):
     r = subprocess.run([qr_harness_tsan, *args.split()], capture_output=True, text=True, env=env, timeout=1500)
     assert "ok" in r.stdout, (r.stdout + r.stderr)[-3000:]
     assert "WARNING: ThreadSanitizer" not in r.stderr, r.stderr[:4000]
+
+
+def _build_det(tmp_path_factory, sanitizer):
+    if not os.path.exists(CLANG):
+        pytest.skip("host clang++ with sanitizers not available")
+    out = str(tmp_path_factory.mktemp("emu") / f"det_harness_{sanitizer}")
+    cmd = [CLANG, "-std=c++20", "-O1", "-g", f"-fsanitize={sanitizer}", "-fno-omit-frame-pointer",
+           "-I", os.path.join(ROOT, "include"), os.path.join(HERE, "cpu_emu", "det_harness.cpp"),
+           "-o", out, "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    return out
+
+
+@pytest.mark.parametrize("args", ["5 6 2", "32 2 1"])
+def test_det_sweep_in_bounds_and_matches_host(tmp_path_factory, args):
+    """The determinant-mode kernel on uneven chains (N - c sites, L - c slices; N = 32 is the
+    largest it takes): in bounds, and the field, accept count and measured inv(M(0)) equal a
+    plain host restatement on the same Philox uniforms."""
+    exe = _build_det(tmp_path_factory, "address")
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
+    r = subprocess.run([exe, *args.split()], capture_output=True, text=True, env=env, timeout=600)
+    assert r.returncode == 0 and "ok" in r.stdout, (r.stdout + r.stderr)[-3000:]
+
+
+def test_det_sweep_race_free(tmp_path_factory):
+    exe = _build_det(tmp_path_factory, "thread")
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=0 report_signal_unsafe=0")
+    r = subprocess.run([exe, "6", "3", "2"], capture_output=True, text=True, env=env, timeout=900)
+    assert "ok" in r.stdout, (r.stdout + r.stderr)[-3000:]
+    assert "WARNING: ThreadSanitizer" not in r.stderr, r.stderr[:4000]

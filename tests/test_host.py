@@ -74,3 +74,14 @@ def test_jackknife_and_structure_factor():
     c = np.outer(s, s)
     assert abs(tools.structure_factor(c, pos, (np.pi, np.pi)) - 16.0) < 1e-12
     assert abs(tools.structure_factor(c, pos, (0.0, 0.0))) < 1e-12
+
+
+def test_det_mode_rejects_fp32_and_delay_before_device_use():
+    """Option checks run before any device call, so they hold on a host without a GPU."""
+    from latticeqmc_amd._native import LqmcError
+    from latticeqmc_amd.engine import ChainSpec, SweepEngine
+    k = np.zeros((4, 4))
+    spec = ChainSpec(k, 1.0, 4, 4.0, np.ones((4, 4), dtype=np.int8), expk=np.eye(4), expk_inv=np.eye(4), lam=0.5)
+    for kw in ({"precision": "fp32"}, {"delay": 4}):
+        with pytest.raises(LqmcError, match="determinant mode"):
+            SweepEngine([spec], mode="det", rng="replay", **kw)
