@@ -49,6 +49,7 @@ enum {
   LQMC_FLAG_ZERO_PIVOT = 2,     /* QR or LU met a zero pivot */
   LQMC_FLAG_REPLAY_UNDERRUN = 4,/* a replay sweep ran past the uploaded uniforms */
   LQMC_FLAG_NAN_G = 8,          /* a recompute found NaN/Inf in G */
+  LQMC_FLAG_FLIP_BARRIER = 16,  /* a wait between the workgroups of one chain's flip timed out */
 };
 
 typedef struct {
@@ -62,6 +63,8 @@ typedef struct {
   int32_t trace;       /* 1: keep per-attempt ratio/accept of the last sweep */
   int32_t obs_capacity;/* per-chain ring of per-sweep observables (0 = 4096) */
   int32_t flip_threads;/* threads per flip workgroup (0 = automatic) */
+  int32_t flip_groups; /* workgroups per chain in the delayed flip, delay 16 or 32 only
+                          (0 = automatic: fill the compute units; 1 = one per chain) */
 } lqmc_options;
 
 typedef struct {

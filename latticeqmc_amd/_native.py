@@ -27,7 +27,8 @@ EXPORTS = (
 class Options(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("precision", ctypes.c_int32), ("delay", ctypes.c_int32),
                 ("n_stab", ctypes.c_int32), ("rng", ctypes.c_int32), ("trace", ctypes.c_int32),
-                ("obs_capacity", ctypes.c_int32), ("flip_threads", ctypes.c_int32)]
+                ("obs_capacity", ctypes.c_int32), ("flip_threads", ctypes.c_int32),
+                ("flip_groups", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):

@@ -147,6 +147,9 @@ struct FlipArgs {
   int rng;
   int trace;
   int delay;
+  int wpc;        // workgroups per chain of the delayed kernel (0 or 1: one)
+  unsigned* bar;  // [nchains][2] barrier counters (wpc > 1), zero between launches
+  int nchains;
   int pad;
 };
 

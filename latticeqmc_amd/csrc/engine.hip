@@ -38,7 +38,8 @@ void launch_expv(const ChainDev* chains, int n, hipStream_t s);
 void launch_flip(int fp32, int delayed, const FlipArgs& a, int nchains, int threads, size_t shmem,
                  hipStream_t s);
 size_t flip_delayed_shmem(int nmax, int k, int threads, int fp32);
-size_t flip_delayed_workspace(int n, int k, int fp32);
+size_t flip_delayed_workspace(int n, int k, int fp32, int groups);
+int flip_grid(int nchains, int groups);
 int flip_delayed_threads_limit(int k);
 size_t qr_blocked_shmem(int nmax);
 size_t det_sweep_shmem(int nmax, int lmax);
@@ -134,6 +135,8 @@ struct lqmc_ctx {
   int* d_eye_ns = nullptr;
   int nmax = 0, lmax = 0;
   int flip_threads = 256;
+  int flip_groups = 1;           // workgroups per chain of the delayed flip
+  unsigned* d_bar = nullptr;     // [nchains][2] their barrier counters
   size_t flip_shmem = 0;
   size_t det_shmem = 0;
   // graphs
@@ -185,7 +188,7 @@ static void layout_chain(const lqmc_ctx* x, Chain& c, char* base, size_t& off) {
   carve(base, off, c.ones, N);
   if (x->opt.delay > 1) {
     char* w = nullptr;
-    carve(base, off, w, flip_delayed_workspace(c.N, x->opt.delay, fp32 ? 1 : 0));
+    carve(base, off, w, flip_delayed_workspace(c.N, x->opt.delay, fp32 ? 1 : 0, x->flip_groups));
     c.dws = w;
   }
   for (int s = 0; s < 2; ++s) {
@@ -586,7 +589,8 @@ static void run_ops(lqmc_ctx* x, const std::vector<Op>& ops, int measure, hipStr
         launch_det_sweep(x->d_chains, x->d_state, x->nchains, measure, x->opt.rng, x->opt.trace, x->det_shmem, s);
         break;
       case OP_FLIP: {
-        FlipArgs a{x->d_chains, x->d_state, op.t, measure, x->opt.rng, x->opt.trace, x->opt.delay, 0};
+        FlipArgs a{x->d_chains, x->d_state, op.t, measure, x->opt.rng, x->opt.trace, x->opt.delay,
+                   x->flip_groups, x->d_bar, x->nchains, 0};
         launch_flip(fp32, x->opt.delay > 1, a, x->nchains, x->flip_threads, x->flip_shmem, s);
         break;
       }
@@ -670,6 +674,41 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
     delete x;
     return fail(LQMC_EINVAL, buf);
   }
+  // Workgroups per chain of the delayed flip (blocks of 16 or 32 sites, whole 16-row strips):
+  // all of them must be resident at once, since they wait for each other between blocks, so
+  // the grid stays within one workgroup per compute unit.
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+      delete x;
+      return fail(LQMC_EHIP, "hipGetDeviceProperties failed");
+    }
+    const int cus = prop.multiProcessorCount;
+    const int padded = 8 * ((n_chains + 7) / 8);
+    const bool splittable = o.mode == LQMC_MODE_DQMC && (o.delay == 16 || o.delay == 32);
+    const int waves = x->flip_threads / 64;
+    if (o.flip_groups < 0 || o.flip_groups > 64) {
+      delete x;
+      return fail(LQMC_EINVAL, "flip_groups must be in 0..64");
+    }
+    if (o.flip_groups > 1 && !splittable) {
+      delete x;
+      return fail(LQMC_EINVAL, "flip_groups > 1 needs the Mode D delayed flip with delay 16 or 32");
+    }
+    if (o.flip_groups > 1 && flip_grid(n_chains, o.flip_groups) > cus) {
+      char buf[160];
+      snprintf(buf, sizeof buf, "flip_groups %d: %d workgroups exceed the %d compute units", o.flip_groups,
+               flip_grid(n_chains, o.flip_groups), cus);
+      delete x;
+      return fail(LQMC_EINVAL, buf);
+    }
+    if (o.flip_groups > 0) x->flip_groups = o.flip_groups;
+    else if (splittable) {
+      // at least one 16-row strip of either spin per wavefront
+      const int strips = 2 * ((x->nmax + 15) / 16);
+      x->flip_groups = std::max(1, std::min(cus / padded, strips / waves));
+    } else x->flip_groups = 1;
+  }
   if (o.mode == LQMC_MODE_DET) {
     x->det_shmem = det_sweep_shmem(x->nmax, x->lmax);
     if (x->det_shmem > (size_t)maxsh) {
@@ -696,6 +735,8 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
   if (e == hipSuccess) e = hipMemset(x->d_state, 0, sizeof(ChainState) * n_chains);
   if (e == hipSuccess) e = hipMalloc(&x->d_eye_ptrs, sizeof(double*) * n_chains);
   if (e == hipSuccess) e = hipMalloc(&x->d_eye_ns, sizeof(int) * n_chains);
+  if (e == hipSuccess) e = hipMalloc(&x->d_bar, sizeof(unsigned) * 2 * n_chains);
+  if (e == hipSuccess) e = hipMemset(x->d_bar, 0, sizeof(unsigned) * 2 * n_chains);
   if (e != hipSuccess) {
     std::string m = std::string("allocation failed: ") + hipGetErrorString(e);
     lqmc_destroy(x);
@@ -1064,6 +1105,7 @@ int lqmc_destroy(lqmc_ctx* x) {
   if (x->d_state) (void)hipFree(x->d_state);
   if (x->d_eye_ptrs) (void)hipFree(x->d_eye_ptrs);
   if (x->d_eye_ns) (void)hipFree(x->d_eye_ns);
+  if (x->d_bar) (void)hipFree(x->d_bar);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x;
   return LQMC_OK;

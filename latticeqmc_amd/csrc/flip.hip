@@ -286,8 +286,9 @@ __device__ inline void wave_sync() {
 //     the unit-triangular inverses and scatter Mx = C (I + U C)^-1 and
 //     My = (I - L)^-1 onto the block's rows (k x m, zero on rejected sites);
 //  4. G <- G0 + P Z with P = -G0[:, B] Mx and Z = My^T (E_B - G0[B, :]) on the matrix
-//     cores (= G0 - sum_j x_j y_j^T), block_update below.
-// Rank-m update of both spins' G by an accepted block of sites B = b0 .. b0+kb-1:
+//     cores (= G0 - sum_j x_j y_j^T), form_z and update_strip below.
+// Rank-m update of both spins' G by an accepted block of sites B = b0 .. b0+kb-1 (form_z,
+// then update_strip over the strips):
 //   G_s += P_s Z_s,  P_s = -G0_s[:, B] Mx_s (N x m),  Z_s = My_s^T (E_B - G0_s[B, :]) (m x N),
 // with Mx_s, My_s the k x m factors scattered on the block rows (LDS, leading dimension K).
 // In the 16x16x4 matrix-core layout, lane l's accumulator element v of a 16 x 16 product
@@ -303,94 +304,171 @@ __device__ inline void wave_sync() {
 //    issued before its first matrix-core op.
 // Phase 1 reads rows B of G0 before any strip is updated; phase 2 reads G0[strip, B] and
 // updates only its own strip.
-template <typename T, int K, int RU>
-__device__ void block_update(T* G, size_t n2, int N, int Np, int b0, int kb, int m, const double* Mx,
-                             const double* My, T* zws) {
+template <typename T, int K>
+__device__ void form_z(const T* G, size_t n2, int N, int Np, int b0, int kb, int m, const double* My, T* zws) {
   using acc_t = typename MfmaOp<T>::acc_t;
   constexpr int JT = (K + 15) / 16;  // 16-wide tiles over the accepts
+  constexpr int KS = K / 4;           // k-steps of one tile
+  constexpr int TB = K >= 32 ? 2 : 4; // tiles per wavefront whose loads are issued together
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int nct = Np >> 4;
   const int jtu = (m + 15) >> 4;
-  LQMC_GLOBAL T* Gg = gptr(G);
+  const int ntile = 2 * nct * JT;
+  const LQMC_GLOBAL T* Gg = gptr(G);
   LQMC_GLOBAL T* Zg = gptr(zws);
-  for (int t = wave; t < 2 * nct * JT; t += nw) {
-    const int sp = t / (nct * JT), ct = (t / JT) % nct, jt = t % JT;
-    if (jt >= jtu) continue;
-    const double* Mys = My + sp * K * K;
-    const LQMC_GLOBAL T* Gs = Gg + sp * n2;
-    const int c = (ct << 4) + li, a = (jt << 4) + li;
-    acc_t z = acc_t{0, 0, 0, 0};
-    for (int kk = 0; kk < kb; kk += 4) {
-      const int q = kk + lk;
-      const T x = (T)ld0(Mys, (size_t)q * K + a, q < kb && a < m);
-      T y = -ldg0(Gs, (size_t)(b0 + q) * N + c, q < kb && c < N);
-      if (q < kb && c == b0 + q) y = y + T(1);
-      z = MfmaOp<T>::mma(x, y, z);
-    }
-    LQMC_GLOBAL T* zo = Zg + (size_t)t * 256 + lane;
+  for (int t0 = wave; t0 < ntile; t0 += TB * nw) {
+    T xv[TB][KS], yv[TB][KS];
+    // every load of the group first (clamped addresses, masked values), then the products
 #pragma unroll
-    for (int v = 0; v < 4; ++v) zo[v * 64] = z[v];
+    for (int u = 0; u < TB; ++u) {
+      const int t = t0 + u * nw;
+      const int sp = (t / (nct * JT)) & 1, ct = (t / JT) % nct, jt = t % JT;
+      const bool ok = t < ntile && jt < jtu;
+      const double* Mys = My + sp * K * K;
+      const LQMC_GLOBAL T* Gs = Gg + sp * n2;
+      const int c = (ct << 4) + li, a = (jt << 4) + li;
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const int q = 4 * ks + lk;
+        xv[u][ks] = (T)ld0(Mys, (size_t)q * K + a, ok && q < kb && a < m);
+        T y = -ldg0(Gs, (size_t)(b0 + q) * N + c, ok && q < kb && c < N);
+        if (ok && q < kb && c == b0 + q) y = y + T(1);
+        yv[u][ks] = y;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < TB; ++u) {
+      const int t = t0 + u * nw;
+      if (t >= ntile || t % JT >= jtu) continue;
+      acc_t z = acc_t{0, 0, 0, 0};
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) z = MfmaOp<T>::mma(xv[u][ks], yv[u][ks], z);
+      LQMC_GLOBAL T* zo = Zg + (size_t)t * 256 + lane;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) zo[v * 64] = z[v];
+    }
+  }
+}
+
+// Phase 2 for one 16-row strip rs (spin rs / nct, rows (rs % nct) * 16 ..) of G, by one wavefront.
+template <typename T, int K, int RU>
+__device__ void update_strip(T* G, size_t n2, int N, int Np, int b0, int kb, int m, const double* Mx,
+                             const T* zws, int rs) {
+  using acc_t = typename MfmaOp<T>::acc_t;
+  constexpr int JT = (K + 15) / 16;
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nct = Np >> 4;
+  const int jtu = (m + 15) >> 4;
+  LQMC_GLOBAL T* Gg = gptr(G);
+  const LQMC_GLOBAL T* Zg = gptr(zws);
+  const int sp = rs / nct, r0 = (rs % nct) << 4;
+  const double* Mxs = Mx + sp * K * K;
+  LQMC_GLOBAL T* Gs = Gg + sp * n2;
+  const int r = r0 + li;
+  acc_t pt[JT];
+#pragma unroll
+  for (int jt = 0; jt < JT; ++jt) {
+    pt[jt] = acc_t{0, 0, 0, 0};
+    if (jt < jtu) {
+      const int a = (jt << 4) + li;
+      for (int kk = 0; kk < kb; kk += 4) {
+        const int q = kk + lk;
+        const T x = (T)ld0(Mxs, (size_t)q * K + a, q < kb && a < m);
+        const T y = ldg0(Gs, (size_t)r * N + b0 + q, r < N && q < kb);
+        pt[jt] = MfmaOp<T>::mma(x, y, pt[jt]);
+      }
+      pt[jt] = -pt[jt];
+    }
+  }
+  for (int c0 = 0; c0 < nct; c0 += RU) {
+    acc_t acc[RU];
+    T zv[RU][JT][4];
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+      const int ct = c0 + u;
+      const int col = (ct << 4) + li;
+      const bool ok = ct < nct && col < N;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = r0 + MfmaOp<T>::row(lane, v);
+        acc[u][v] = ldg0(Gs, (size_t)row * N + col, ok && row < N);
+      }
+#pragma unroll
+      for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          zv[u][jt][v] = ldg0(Zg, ((size_t)((sp * nct + ct) * JT + jt) * 4 + v) * 64 + lane,
+                              ct < nct && jt < jtu);
+    }
+#pragma unroll
+    for (int u = 0; u < RU; ++u)
+#pragma unroll
+      for (int jt = 0; jt < JT; ++jt)
+        if (jt < jtu) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[u] = MfmaOp<T>::mma(pt[jt][v], zv[u][jt][v], acc[u]);
+        }
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+      const int ct = c0 + u;
+      const int col = (ct << 4) + li;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = r0 + MfmaOp<T>::row(lane, v);
+        if (ct < nct && row < N && col < N) Gs[(size_t)row * N + col] = acc[u][v];
+      }
+    }
+  }
+}
+
+// ---- several workgroups per chain ------------------------------------------------------------
+// With W > 1 workgroups on one chain (blocks of 16 or 32 sites, so a block is whole 16-row
+// strips), every workgroup runs the serial phase, the inverses and the Z tiles of each block
+// itself (from the same G rows and uniforms, so with identical results), and the 2 ceil(N/16)
+// strips of both spins are shared out over the W x (waves per workgroup) wavefronts.  Between
+// blocks the W workgroups meet at a per-chain barrier.  The strips holding the block's own
+// rows B are read by every workgroup during the block (serial phase and Z), so their owner
+// defers their update until after the next barrier, when no workgroup reads them any more
+// (later blocks read only their own rows B' and each strip owner its own rows).
+//
+// Barrier: counter bar[0] counts arrivals within one launch (the n-th barrier waits for n W),
+// bar[1] counts exits; the last workgroup to exit resets both, so each launch starts from
+// zero.  A wait is bounded by FLIP_BARRIER_SPINS polls; on expiry the workgroup records
+// flag 16, stops waiting for the rest of the launch and still arrives, so every wave exits.
+constexpr unsigned FLIP_BARRIER_SPINS = 1u << 22;
+
+__device__ inline void chain_barrier(unsigned* bar, unsigned target, ChainState* st, bool& broken) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // release the workgroup's G stores (ordered before this point by the barrier above) at
+    // device scope, arrive, wait, then acquire
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    atomicAdd(bar, 1u);
+    if (!broken) {
+      unsigned spins = 0;
+      while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins >= FLIP_BARRIER_SPINS) {
+          broken = true;
+          atomicOr(&st->flags, 16u);
+          break;
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
-  for (int rs = wave; rs < 2 * nct; rs += nw) {
-    const int sp = rs / nct, r0 = (rs % nct) << 4;
-    const double* Mxs = Mx + sp * K * K;
-    LQMC_GLOBAL T* Gs = Gg + sp * n2;
-    const int r = r0 + li;
-    acc_t pt[JT];
-#pragma unroll
-    for (int jt = 0; jt < JT; ++jt) {
-      pt[jt] = acc_t{0, 0, 0, 0};
-      if (jt < jtu) {
-        const int a = (jt << 4) + li;
-        for (int kk = 0; kk < kb; kk += 4) {
-          const int q = kk + lk;
-          const T x = (T)ld0(Mxs, (size_t)q * K + a, q < kb && a < m);
-          const T y = ldg0(Gs, (size_t)r * N + b0 + q, r < N && q < kb);
-          pt[jt] = MfmaOp<T>::mma(x, y, pt[jt]);
-        }
-        pt[jt] = -pt[jt];
-      }
-    }
-    for (int c0 = 0; c0 < nct; c0 += RU) {
-      acc_t acc[RU];
-      T zv[RU][JT][4];
-#pragma unroll
-      for (int u = 0; u < RU; ++u) {
-        const int ct = c0 + u;
-        const int col = (ct << 4) + li;
-        const bool ok = ct < nct && col < N;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = r0 + MfmaOp<T>::row(lane, v);
-          acc[u][v] = ldg0(Gs, (size_t)row * N + col, ok && row < N);
-        }
-#pragma unroll
-        for (int jt = 0; jt < JT; ++jt)
-#pragma unroll
-          for (int v = 0; v < 4; ++v)
-            zv[u][jt][v] = ldg0(Zg, ((size_t)((sp * nct + ct) * JT + jt) * 4 + v) * 64 + lane,
-                                ct < nct && jt < jtu);
-      }
-#pragma unroll
-      for (int u = 0; u < RU; ++u)
-#pragma unroll
-        for (int jt = 0; jt < JT; ++jt)
-          if (jt < jtu) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[u] = MfmaOp<T>::mma(pt[jt][v], zv[u][jt][v], acc[u]);
-          }
-#pragma unroll
-      for (int u = 0; u < RU; ++u) {
-        const int ct = c0 + u;
-        const int col = (ct << 4) + li;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = r0 + MfmaOp<T>::row(lane, v);
-          if (ct < nct && row < N && col < N) Gs[(size_t)row * N + col] = acc[u][v];
-        }
-      }
+}
+
+__device__ inline void chain_exit(unsigned* bar, int W) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(bar + 1, 1u) == (unsigned)(W - 1)) {
+      atomicExch(bar, 0u);
+      atomicExch(bar + 1, 0u);
     }
   }
 }
@@ -399,20 +477,41 @@ __device__ void block_update(T* G, size_t n2, int N, int Np, int b0, int kb, int
 // k x k work in registers that only fit the 256-VGPR budget of a 512-thread workgroup.
 constexpr int flip_delayed_max_threads(int k) { return k >= 16 ? 512 : 1024; }
 
+// Block index -> (chain, workgroup of the chain).  W > 1: the W workgroups of a chain get block
+// indices of equal residue mod 8, i.e. the same XCD under round-robin dispatch, so the rows
+// they share stay in one L2; blocks past the last chain return at once.
+__device__ inline void flip_block_map(const FlipArgs& a, int& c, int& w) {
+  const int W = a.wpc > 1 ? a.wpc : 1;
+  if (W == 1) {
+    c = blockIdx.x;
+    w = 0;
+    return;
+  }
+  const int b = blockIdx.x, slot = b >> 3;
+  c = (b & 7) + 8 * (slot / W);
+  w = slot % W;
+}
+
 template <typename T, int K>
 __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(FlipArgs a) {
   static_assert(K >= 4 && K <= 32 && K % 4 == 0, "block size");
-  const int c = blockIdx.x;
+  int c, w;
+  flip_block_map(a, c, w);
+  const int W = a.wpc > 1 ? a.wpc : 1;
+  if (W > 1 && c >= a.nchains) return;
   const ChainDev ch = a.chains[c];
   if (a.t >= ch.L) return;
   const int N = ch.N;
   const int l = ch.L - 1 - a.t;
   const int Np = (N + 15) & ~15;
+  const int nct = Np >> 4;
   const size_t n2 = (size_t)N * N;
   T* __restrict__ G = static_cast<T*>(ch.G);
   int8_t* fld = ch.field + (size_t)l * N;
-  // workspace: the Z tiles of both spins in register order (2 x K16 x Np, form precision)
-  T* wsT = static_cast<T*>(ch.dws);
+  // workspace: the Z tiles of both spins in register order (2 x K16 x Np, form precision),
+  // one per workgroup of the chain
+  T* wsT = static_cast<T*>(ch.dws) + (size_t)w * 2 * ((K + 15) & ~15) * Np;
+  unsigned* bar = a.bar + 2 * c;
 
   LQMC_DYNAMIC_LDS(double, sm);
   double* red = sm;                        // blockDim (epilogue reductions)
@@ -422,18 +521,43 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
   double* ub = cf + 2 * K;                 // [K] uniforms of the block
   int* aidx = reinterpret_cast<int*>(ub + K);  // [K] block-local site of each accept
   int* sb = aidx + K;                          // [K] field entries of the block
-  __shared__ int m_sh;
+  int* m_sh = sb + K;                          // [4] accept count of the block
+  double* Mxp = reinterpret_cast<double*>(m_sh + 4);  // [2][K][K] Mx of the previous block (W > 1)
+  int8_t* fl = reinterpret_cast<int8_t*>(Mxp + 2 * K * K);  // [N] the slice's field (flipped here)
 
   const unsigned long long sweep = a.st[c].sweep;
   unsigned long long nacc = 0;
   bool underrun = false, bad = false;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int gw = w * nw + wave, tw = W * nw;  // wavefront index within the chain, and count
+  for (int i = threadIdx.x; i < N; i += blockDim.x) fl[i] = fld[i];
+  __syncthreads();
+  bool pending = false, broken = false;
+  int pb0 = 0, pkb = 0, pm = 0;
+  unsigned nbar = 0;
+  // The deferred strips of the previous block (its own rows, after the chain barrier), with
+  // that block's Mx kept in Mxp and its Z tiles still in the workspace.  Each is applied by
+  // its owner workgroup; while wave 0 runs the next serial phase the strips go to waves 1 ..
+  auto deferred_strips = [&](bool serial_busy) {
+    const int s0 = pb0 >> 4, s1 = (pb0 + pkb - 1) >> 4;
+    int i = 0;
+    for (int sp = 0; sp < 2; ++sp)
+      for (int st = s0; st <= s1; ++st, ++i) {
+        const int rs = sp * nct + st, own = rs % tw;
+        if (own / nw != w) continue;
+        const int dw = serial_busy ? 1 + i % (nw - 1) : own % nw;
+        if (wave == dw) update_strip<T, K, (K >= 16 ? 4 : 2)>(G, n2, N, Np, pb0, pkb, pm, Mxp, wsT, rs);
+      }
+  };
 
   for (int b0 = 0; b0 < N; b0 += K) {
     const int kb = min(K, N - b0);
+    if constexpr (K >= 16) {
+      if (pending) chain_barrier(bar, ++nbar * (unsigned)W, &a.st[c], broken);
+    }
     if (threadIdx.x < kb) {
       const int i = b0 + threadIdx.x;
-      sb[threadIdx.x] = fld[i];
+      sb[threadIdx.x] = fl[i];
       bool und = false;
       ub[threadIdx.x] = draw_uniform(ch, &a.st[c], c, a.rng, sweep, a.t, i, und);
       if (und) underrun = true;
@@ -456,57 +580,88 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
         const int p = h * R + i;
         col[i] = (live && p < kb) ? (double)G[sp * n2 + (size_t)(b0 + p) * N + b0 + q] : 0.0;
       }
+      // the block's field entries and uniforms, lane jj holding site jj (picked by readlane)
+      const int sb_l = sb[lane < kb ? lane : 0];
+      const double ub_l = ub[lane < kb ? lane : 0];
+      // One site jj (hj = jj / R) with cj = G(current)[jj][q] of my spin on the lanes with
+      // h == hj.  For R <= 8 the sites of one hj are unrolled, so cj is a register of col;
+      // otherwise it is picked by a select chain.
       int m = 0;
-#pragma unroll 1
-      for (int jj = 0; jj < kb; ++jj) {
-        const int hj = jj / R, ij = jj % R;
-        double cj = 0.0;  // G(current)[jj][q] of my spin, on the lanes with h == hj
-#pragma unroll
-        for (int i = 0; i < R; ++i)
-          if (i == ij) cj = col[i];
+      auto site = [&](const int jj, const int hj, const double cj) {
         const double gu = readlane_f64(cj, hj * K + jj);
         const double gd = readlane_f64(cj, K * H + hj * K + jj);
-        const int s = sb[jj];
+        const int s = __builtin_amdgcn_readlane(sb_l, jj);
         const double e2p = ch.e2p, e2m = ch.e2m;
         const double ep = s > 0 ? e2p : e2m;
         const double em = s > 0 ? e2m : e2p;
         const double d_up = 1.0 + (1.0 - gu) * (ep - 1.0);
         const double d_dn = 1.0 + (1.0 - gd) * (em - 1.0);
         const double ratio = d_up * d_dn;
-        const bool acc = ub[jj] <= ratio;
+        const bool acc = readlane_f64(ub_l, jj) <= ratio;
         if (!isfinite(ratio)) bad = true;
-        if (a.trace && lane == 0) {
+        if (a.trace && lane == 0 && w == 0) {
           ch.tr_ratio[(size_t)a.t * N + b0 + jj] = ratio;
           ch.tr_acc[(size_t)a.t * N + b0 + jj] = acc ? 1 : 0;
         }
-        if (acc) {
-          const double cu = (ep - 1.0) / d_up, cd = (em - 1.0) / d_dn;
-          const double cfs = sp == 0 ? cu : cd;
-          // y = e_jj - G[jj, :]^T: column q's entry sits on lane (sp, hj, q)
-          double yq = -__shfl(cj, sp * K * H + hj * K + q, 64);
-          if (q == jj) yq = yq + 1.0;
-          const int src = sp * K * H + h * K + jj;  // my rows of x = column jj of my spin
+        // The exchanges and the coefficient do not depend on the decision: they are
+        // issued before it, so their latencies overlap.  My spin's coefficient is
+        // c = (e^{+-2 lambda s} - 1) / d (one division per lane); y = e_jj - G[jj, :]^T has
+        // column q's entry on lane (sp, hj, q); my rows of x = column jj of my spin come
+        // from lane src.
+        const int src = sp * K * H + h * K + jj;
+        double yq = -__shfl(cj, sp * K * H + hj * K + q, 64);
+        double xp[R];
 #pragma unroll
-          for (int i = 0; i < R; ++i) {
-            const int p = h * R + i;
-            const double xp = __shfl(col[i], src, 64);
-            if (lane == src && p < kb) Xsb[sp * K * K + p * K + m] = cfs * xp;
-            if (live && p < kb) col[i] -= cfs * (xp * yq);
+        for (int i = 0; i < R; ++i) xp[i] = __shfl(col[i], src, 64);
+        const double num = sp == 0 ? ep - 1.0 : em - 1.0;
+        const double cfs = num / (sp == 0 ? d_up : d_dn);
+        if (acc) {
+          if (q == jj) yq = yq + 1.0;
+          // G <- G - c x y^T on the k x k block.  Entries of rows >= kb and of lanes past
+          // the block are never read (the pivots and exchanges come from lanes q = jj < kb,
+          // rows < kb), so the update runs unmasked.
+          const double cy = cfs * yq;
+#pragma unroll
+          for (int i = 0; i < R; ++i) col[i] = fma(-cy, xp[i], col[i]);
+          if (lane == src) {
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+              if (h * R + i < kb) Xsb[sp * K * K + (h * R + i) * K + m] = cfs * xp[i];
           }
           if (live && h == 0) Ysb[sp * K * K + q * K + m] = yq;
           if (lane == 0) {
-            cf[m] = cu;
-            cf[K + m] = cd;
+            cf[m] = cfs;
             aidx[m] = jj;
-            fld[b0 + jj] = (int8_t)(-s);
+            fl[b0 + jj] = (int8_t)(-s);
           }
+          if (lane == K * H) cf[K + m] = cfs;
           ++m;
         }
+      };
+      if constexpr (R <= 8) {
+#pragma unroll 1
+        for (int hj = 0; hj * R < kb; ++hj) {
+#pragma unroll
+          for (int ij = 0; ij < R; ++ij)
+            if (hj * R + ij < kb) site(hj * R + ij, hj, col[ij]);
+        }
+      } else {
+#pragma unroll 1
+        for (int jj = 0; jj < kb; ++jj) {
+          double cj = 0.0;
+#pragma unroll
+          for (int i = 0; i < R; ++i)
+            if (i == jj % R) cj = col[i];
+          site(jj, jj / R, cj);
+        }
       }
-      if (lane == 0) m_sh = m;
+      if (lane == 0) m_sh[0] = m;
+    } else if (K >= 16 && pending) {
+      deferred_strips(true);
     }
     __syncthreads();
-    const int m = m_sh;
+    pending = false;
+    const int m = m_sh[0];
     if (m == 0) continue;
     nacc += m;
     // The coefficients of the two unit upper-triangular systems per spin, in accept
@@ -548,14 +703,26 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
       const double* Tw = Xsb + isp * 2 * K * K + (lower ? K * K : 0);
 #pragma unroll
       for (int p = 0; p < K; ++p) inv[p] = p == j ? 1.0 : 0.0;
+      // row r of the table is read while row r+1 is being used (one LDS wait per row)
+      double trow[K];
+#pragma unroll
+      for (int p = 0; p < K; ++p) trow[p] = Tw[(K - 2) * K + p];
 #pragma unroll 1
       for (int r = K - 2; r >= 0; --r) {
-        double v = 0.0;
+        double tnext[K];
+        const int rn = r > 0 ? r - 1 : 0;
 #pragma unroll
-        for (int p = 0; p < K; ++p) v -= Tw[r * K + p] * inv[p];
+        for (int p = 0; p < K; ++p) tnext[p] = Tw[rn * K + p];
+        // four interleaved partial sums shorten the dependent FMA chain
+        double v4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int p = 0; p < K; ++p) v4[p & 3] -= trow[p] * inv[p];
+        const double v = (v4[0] + v4[1]) + (v4[2] + v4[3]);
 #pragma unroll
         for (int p = 0; p < K; ++p)
           if (p == r && r < j) inv[p] = v;
+#pragma unroll
+        for (int p = 0; p < K; ++p) trow[p] = tnext[p];
       }
     }
     __syncthreads();
@@ -569,10 +736,41 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
         if (r <= j) out[aidx[r] * K + j] = lower ? inv[r] : cs[r] * inv[r];
     }
     __syncthreads();
-    block_update<T, K, (K >= 16 ? 4 : 2)>(G, n2, N, Np, b0, kb, m, Xsb, Ysb, wsT);
+    form_z<T, K>(G, n2, N, Np, b0, kb, m, Ysb, wsT);
+    __syncthreads();
+    {
+      // W > 1: the strips of this block's rows wait for the next barrier (see above)
+      const bool defer = K >= 16 && W > 1;
+      const int s0 = defer ? b0 >> 4 : nct, s1 = defer ? (b0 + kb - 1) >> 4 : -1;
+      for (int rs = gw; rs < 2 * nct; rs += tw) {
+        const int st = rs % nct;
+        if (st < s0 || st > s1) update_strip<T, K, (K >= 16 ? 4 : 2)>(G, n2, N, Np, b0, kb, m, Xsb, wsT, rs);
+      }
+    }
+    if (K >= 16 && W > 1) {
+      pending = true;
+      pb0 = b0;
+      pkb = kb;
+      pm = m;
+      for (int e = threadIdx.x; e < 2 * K * K; e += blockDim.x) Mxp[e] = Xsb[e];
+    }
     __syncthreads();
   }
-  slice_epilogue<T>(a, ch, c, l, G, nacc, underrun, bad, red);
+  if constexpr (K >= 16) {
+    if (pending) {
+      chain_barrier(bar, ++nbar * (unsigned)W, &a.st[c], broken);
+      deferred_strips(false);
+      __syncthreads();
+    }
+  }
+  // On slice 0 every workgroup has read the sweep counter and (when measuring) every strip is
+  // final before workgroup 0 measures and advances the counter.
+  if (W > 1 && l == 0) chain_barrier(bar, ++nbar * (unsigned)W, &a.st[c], broken);
+  if (w == 0) {
+    for (int i = threadIdx.x; i < N; i += blockDim.x) fld[i] = fl[i];
+    slice_epilogue<T>(a, ch, c, l, G, nacc, underrun, bad, red);
+  }
+  if (W > 1) chain_exit(bar, W);
 }
 
 // ---- determinant mode ------------------------------------------------------------------------
@@ -788,25 +986,31 @@ void launch_det_sweep(const ChainDev* chains, ChainState* st, int nchains, int m
 int flip_delayed_threads_limit(int k) { return flip_delayed_max_threads(k); }
 
 size_t flip_delayed_shmem(int nmax, int k, int threads, int fp32) {
-  (void)nmax;
   (void)fp32;
-  return sizeof(double) * (threads + 4 * (size_t)k * k + 3 * (size_t)k) + sizeof(int) * 2 * k;
+  return sizeof(double) * (threads + 6 * (size_t)k * k + 3 * (size_t)k) + sizeof(int) * (2 * k + 4) + (size_t)nmax;
 }
 
-// Bytes of the per-chain delayed-update workspace (P and Z of both spins).
-size_t flip_delayed_workspace(int n, int k, int fp32) {
+// Bytes of the per-chain delayed-update workspace (the Z tiles of both spins, one set per
+// workgroup of the chain).
+size_t flip_delayed_workspace(int n, int k, int fp32, int groups) {
   const size_t Np = (n + 15) & ~15;
   const size_t K16 = (k + 15) & ~15;
-  return 2 * K16 * Np * (fp32 ? 4 : 8);
+  return 2 * K16 * Np * (fp32 ? 4 : 8) * (size_t)(groups > 1 ? groups : 1);
+}
+
+// Grid of the delayed kernel: W workgroups per chain, chains padded to a multiple of 8.
+int flip_grid(int nchains, int groups) {
+  return groups > 1 ? 8 * groups * ((nchains + 7) / 8) : nchains;
 }
 
 void launch_flip(int fp32, int delayed, const FlipArgs& a, int nchains, int threads, size_t shmem,
                  hipStream_t s) {
   if (delayed) {
+    const int grid = flip_grid(nchains, a.wpc);
 #define LQMC_DELAYED(KK)                                                                               \
   case KK:                                                                                             \
-    if (fp32) hipLaunchKernelGGL((k_flip_delayed<float, KK>), dim3(nchains), dim3(threads), shmem, s, a); \
-    else hipLaunchKernelGGL((k_flip_delayed<double, KK>), dim3(nchains), dim3(threads), shmem, s, a);    \
+    if (fp32) hipLaunchKernelGGL((k_flip_delayed<float, KK>), dim3(grid), dim3(threads), shmem, s, a); \
+    else hipLaunchKernelGGL((k_flip_delayed<double, KK>), dim3(grid), dim3(threads), shmem, s, a);    \
     break;
     switch (a.delay) {
       LQMC_DELAYED(4)

@@ -64,10 +64,12 @@ class SweepEngine:
     delay     1: rank-1 per accepted flip; k > 1: submatrix blocks of k sites
     n_stab    slices between FP64 recomputes
     rng       "replay" (uniforms supplied per sweep) or "philox" (on device)
+    flip_groups  workgroups per chain in the delayed flip (delay 16 or 32; 0 = automatic,
+              filling the compute units; results do not depend on it)
     """
 
     def __init__(self, chains, mode="dqmc", precision="fp64", delay=1, n_stab=10, rng="replay",
-                 trace=False, device=0, obs_capacity=4096, flip_threads=0, graph=False):
+                 trace=False, device=0, obs_capacity=4096, flip_threads=0, flip_groups=0, graph=False):
         self.lib = nat.load()
         self.chains = list(chains)
         if not self.chains:
@@ -75,7 +77,7 @@ class SweepEngine:
         self.mode, self.precision, self.rng = mode, precision, rng
         self.delay, self.n_stab, self.trace_on = int(delay), int(n_stab), bool(trace)
         opt = nat.Options(_MODES[mode], _PREC[precision], int(delay), int(n_stab), _RNG[rng],
-                          int(bool(trace)), int(obs_capacity), int(flip_threads))
+                          int(bool(trace)), int(obs_capacity), int(flip_threads), int(flip_groups))
         ns = (ctypes.c_int32 * len(self.chains))(*[c.n_sites for c in self.chains])
         ls = (ctypes.c_int32 * len(self.chains))(*[c.time_steps for c in self.chains])
         h = ctypes.c_void_p()

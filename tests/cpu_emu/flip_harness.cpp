@@ -1,7 +1,8 @@
 // Bounds check of the flip kernels in a CPU build (AddressSanitizer): every
 // device buffer is a separate exact-size allocation and the dynamic LDS is an
 // exact-size heap block, so any out-of-range access aborts with a report.
-// usage: flip_harness N L delay threads fp32 nchains t measure
+// usage: flip_harness N L delay threads fp32 nchains t measure [gscale [groups]]
+// groups > 1 runs the delayed kernel with that many workgroups per chain, all blocks at once.
 #include "hip_cpu_shim.h"
 #include "../../latticeqmc_amd/csrc/flip.hip"
 
@@ -22,6 +23,8 @@ int main(int argc, char** argv) {
   std::mt19937_64 rng(1);
   std::uniform_real_distribution<double> U(-0.3, 0.3);
   const double gscale = argc > 9 ? atof(argv[9]) : 1.0;  // off-diagonal scale (larger -> more rejections)
+  const int groups = argc > 10 ? atoi(argv[10]) : 1;
+  if (groups > 1 && K < 16) { std::fprintf(stderr, "groups > 1 needs delay 16 or 32\n"); return 2; }
   std::vector<ChainDev> chains(nch);
   std::vector<ChainState> st(nch);
   for (int c = 0; c < nch; ++c) {
@@ -53,22 +56,27 @@ int main(int argc, char** argv) {
     d.obs_cap = 8;
     d.obs = dalloc<double>(3 * 8);
     if (K > 1) {
-      d.dws = dalloc<char>(flip_delayed_workspace(N, K, fp32));
+      d.dws = dalloc<char>(flip_delayed_workspace(N, K, fp32, groups));
     }
   }
   std::memset(st.data(), 0, sizeof(ChainState) * nch);
-  FlipArgs a{chains.data(), st.data(), t, measure, 1, 1, K, 0};
+  std::vector<unsigned> bar(2 * nch, 0u);
+  FlipArgs a{chains.data(), st.data(), t, measure, 1, 1, K, groups, bar.data(), nch, 0};
   if (K > 1 && (threads > flip_delayed_threads_limit(K) || threads < 256)) {
     std::fprintf(stderr, "threads above the delayed kernel's launch bound %d\n", flip_delayed_threads_limit(K));
     return 2;
   }
   const size_t lds = K > 1 ? flip_delayed_shmem(N, K, threads, fp32) : sizeof(double) * (4 * (size_t)N + threads);
-  std::printf("N=%d L=%d K=%d threads=%d fp32=%d chains=%d t=%d lds=%zu\n", N, L, K, threads, fp32, nch, t, lds);
+  std::printf("N=%d L=%d K=%d threads=%d fp32=%d chains=%d t=%d groups=%d lds=%zu\n", N, L, K, threads, fp32, nch, t,
+              groups, lds);
   std::fflush(stdout);
   if (K > 1) {
 #define RUNK(KK)                                                                          \
   case KK:                                                                                \
-    if (fp32) emu_launch(nch, threads, lds, [&] { k_flip_delayed<float, KK>(a); });       \
+    if (groups > 1) {                                                                     \
+      if (fp32) emu_launch_concurrent(flip_grid(nch, groups), threads, lds, [&] { k_flip_delayed<float, KK>(a); }); \
+      else emu_launch_concurrent(flip_grid(nch, groups), threads, lds, [&] { k_flip_delayed<double, KK>(a); });    \
+    } else if (fp32) emu_launch(nch, threads, lds, [&] { k_flip_delayed<float, KK>(a); }); \
     else emu_launch(nch, threads, lds, [&] { k_flip_delayed<double, KK>(a); });          \
     break;
     switch (K) { RUNK(4) RUNK(8) RUNK(16) RUNK(32) default: std::printf("unsupported K\n"); return 2; }
@@ -78,6 +86,8 @@ int main(int argc, char** argv) {
   }
   unsigned long long acc = 0;
   for (auto& s : st) acc += s.accepted;
+  for (int c = 0; c < nch; ++c)
+    if (st[c].flags) { std::printf("FAIL: flags %u on chain %d\n", st[c].flags, c); return 1; }
   // cross-check against the rank-1 kernel on a copy of the initial state (same uniforms)
   if (K > 1 && argc > 9) {
     std::vector<ChainDev> ref(nch);
@@ -112,6 +122,10 @@ int main(int argc, char** argv) {
     std::printf("vs rank-1: max|dG| %.3e, field differences %d\n", worst, fdiff);
     if (fdiff || worst > (fp32 ? 1e-4 : 1e-11)) { std::printf("FAIL\n"); return 1; }
   }
+  for (int c = 0; c < nch; ++c)
+    if (st[c].flags) { std::printf("FAIL: flags %u on chain %d\n", st[c].flags, c); return 1; }
+  for (unsigned v : bar)
+    if (v) { std::printf("FAIL: barrier counters not reset\n"); return 1; }
   std::printf("ok: accepted %llu of %llu, flags %u\n", acc, (unsigned long long)nch * N, st[0].flags);
   return 0;
 }

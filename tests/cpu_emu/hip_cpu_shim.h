@@ -3,9 +3,12 @@
 // block barrier, the 16x16x4 matrix-core ops emulated across the 64 threads of
 // a wavefront with the lane layout measured on gfx950 (inputs: lane l holds
 // A[l%16][l/16] and B[l/16][l%16]; outputs: column l%16, row l/16 + 4r for
-// FP64 and 4*(l/16) + r for FP32).  Blocks run one after another.
+// FP64 and 4*(l/16) + r for FP32).  emu_launch runs blocks one after another;
+// emu_launch_concurrent runs every block at once (kernels whose workgroups wait
+// for each other).
 #pragma once
 #include <barrier>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -24,8 +27,8 @@
 struct emu_dim3 { unsigned x = 1, y = 1, z = 1; };
 inline thread_local emu_dim3 threadIdx, blockIdx;
 inline emu_dim3 blockDim, gridDim;
-inline char* emu_lds_buf = nullptr;
-inline std::barrier<>* emu_block_barrier = nullptr;
+inline thread_local char* emu_lds_buf = nullptr;
+inline thread_local std::barrier<>* emu_block_barrier = nullptr;
 #define LQMC_DYNAMIC_LDS(type, name) type* name = reinterpret_cast<type*>(emu_lds_buf)
 #define LQMC_GLOBAL
 inline void __syncthreads() { emu_block_barrier->arrive_and_wait(); }
@@ -41,6 +44,14 @@ inline int hipFuncSetAttribute(const void*, int, int) { return 0; }
 
 inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline unsigned atomicOr(unsigned* p, unsigned v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
+inline unsigned atomicAdd(unsigned* p, unsigned v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline unsigned atomicExch(unsigned* p, unsigned v) { return __atomic_exchange_n(p, v, __ATOMIC_SEQ_CST); }
+inline void __threadfence() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
+#define __HIP_MEMORY_SCOPE_AGENT 0
+// polled loads acquire, so the host thread sanitizer sees the barrier's ordering
+#define __hip_atomic_load(p, order, scope) __atomic_load_n((p), __ATOMIC_ACQUIRE)
+// a poll waits 100 us, so the kernels' bounded waits allow minutes of lag between emulated blocks
+#define __builtin_amdgcn_s_sleep(n) std::this_thread::sleep_for(std::chrono::microseconds(100))
 inline unsigned long long atomicMax(unsigned long long* p, unsigned long long v) {
   unsigned long long cur = __atomic_load_n(p, __ATOMIC_SEQ_CST);
   while (cur < v && !__atomic_compare_exchange_n(p, &cur, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {}
@@ -51,7 +62,7 @@ inline long long __double_as_longlong(double d) { long long r; std::memcpy(&r, &
 inline double __longlong_as_double(long long v) { double r; std::memcpy(&r, &v, 8); return r; }
 
 struct EmuWave { double a[64], b[64]; std::barrier<>* bar; };
-inline EmuWave* emu_waves = nullptr;
+inline thread_local EmuWave* emu_waves = nullptr;
 
 template <typename T, typename V>
 inline V emu_mfma(T a, T b, V c, bool fp64) {
@@ -120,25 +131,55 @@ inline int emu_mov_dpp(int v, int ctrl) {
 inline int emu_readlane_i32(int v, int l) { return (int)(long long)__shfl((double)v, l); }
 #define __builtin_amdgcn_readlane(v, l) emu_readlane_i32((v), (l))
 
-// Run fn() as a block of `threads` threads for every block index in [0, blocks).
+// Per-block emulation state: LDS, the block barrier and the wavefront exchange areas.
+struct EmuBlock {
+  char* lds;
+  std::barrier<> bar;
+  std::vector<EmuWave> waves;
+  EmuBlock(int threads, size_t lds_bytes) : bar(threads), waves((threads + 63) / 64) {
+    lds = static_cast<char*>(std::malloc(lds_bytes ? lds_bytes : 1));
+    std::memset(lds, 0xA5, lds_bytes);
+    for (auto& w : waves) w.bar = new std::barrier<>(64);
+  }
+  ~EmuBlock() {
+    for (auto& w : waves) delete w.bar;
+    std::free(lds);
+  }
+};
+
+template <typename F>
+inline void emu_run_blocks(int b_begin, int b_end, int threads, size_t lds_bytes, F& fn) {
+  std::vector<EmuBlock*> blk;
+  for (int b = b_begin; b < b_end; ++b) blk.push_back(new EmuBlock(threads, lds_bytes));
+  std::vector<std::thread> ts;
+  for (int b = b_begin; b < b_end; ++b)
+    for (int t = 0; t < threads; ++t)
+      ts.emplace_back([&, t, b] {
+        EmuBlock* e = blk[b - b_begin];
+        threadIdx.x = t;
+        blockIdx.x = b;
+        emu_lds_buf = e->lds;
+        emu_block_barrier = &e->bar;
+        emu_waves = e->waves.data();
+        fn();
+      });
+  for (auto& t : ts) t.join();
+  for (auto* e : blk) delete e;
+}
+
+// Run fn() as a block of `threads` threads for every block index in [0, blocks), one block
+// after another.
 template <typename F>
 inline void emu_launch(int blocks, int threads, size_t lds_bytes, F fn) {
   blockDim.x = threads;
   gridDim.x = blocks;
-  std::vector<EmuWave> waves((threads + 63) / 64);
-  std::vector<std::barrier<>*> wb;
-  for (auto& w : waves) { w.bar = new std::barrier<>(64); wb.push_back(w.bar); }
-  emu_waves = waves.data();
-  std::barrier<> bar(threads);
-  emu_block_barrier = &bar;
-  for (int b = 0; b < blocks; ++b) {
-    emu_lds_buf = static_cast<char*>(std::malloc(lds_bytes ? lds_bytes : 1));
-    std::memset(emu_lds_buf, 0xA5, lds_bytes);
-    std::vector<std::thread> ts;
-    for (int t = 0; t < threads; ++t)
-      ts.emplace_back([&, t, b] { threadIdx.x = t; blockIdx.x = b; fn(); });
-    for (auto& t : ts) t.join();
-    std::free(emu_lds_buf);
-  }
-  for (auto* p : wb) delete p;
+  for (int b = 0; b < blocks; ++b) emu_run_blocks(b, b + 1, threads, lds_bytes, fn);
+}
+
+// As emu_launch, with every block running at the same time.
+template <typename F>
+inline void emu_launch_concurrent(int blocks, int threads, size_t lds_bytes, F fn) {
+  blockDim.x = threads;
+  gridDim.x = blocks;
+  emu_run_blocks(0, blocks, threads, lds_bytes, fn);
 }

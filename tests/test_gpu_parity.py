@@ -44,12 +44,12 @@ def test_mode_r_matches_reference_fixture(golden_dir, name):
     eng.close()
 
 
-def _run_dqmc_vs_oracle(side, L, beta, u, n_stab, delay, sweeps, seed=0, precision="fp64"):
+def _run_dqmc_vs_oracle(side, L, beta, u, n_stab, delay, sweeps, seed=0, precision="fp64", flip_groups=0):
     k, f = square_case(side, L, beta, u, seed)
     n = side * side
     o = dqmc.DQMCOracle(k, beta, L, u, f, n_stab=n_stab)
     eng = SweepEngine([spec_from(k, beta, L, u, f)], mode="dqmc", precision=precision, delay=delay,
-                      n_stab=n_stab, rng="replay", trace=True)
+                      n_stab=n_stab, rng="replay", trace=True, flip_groups=flip_groups)
     g0 = eng.green(0, which=0)
     init_err = max(np.abs(g0[s] - o.g[s]).max() for s in range(2))
     rs = np.random.RandomState(100 + seed)
@@ -95,6 +95,42 @@ def test_dqmc_fp64_matches_oracle(side, L, beta, n_stab, delay):
     assert worst < 1e-8
     assert st["flags"] == 0
     assert st["max_err"] < 1e-8
+
+
+@pytest.mark.parametrize("side,L,beta,n_stab,delay,groups", [
+    (8, 40, 4.0, 10, 16, 2), (10, 20, 2.0, 10, 16, 3), (6, 20, 2.0, 10, 32, 3),
+])
+def test_dqmc_split_flip_matches_oracle(side, L, beta, n_stab, delay, groups):
+    """The delayed flip with several workgroups per chain (per-block barriers, deferred
+    own-row strips) against the oracle, as the one-workgroup form."""
+    _say(f"[dqmc split] side={side} L={L} delay={delay} groups={groups}")
+    init_err, worst, st = _run_dqmc_vs_oracle(side, L, beta, 4.0, n_stab, delay, sweeps=2, flip_groups=groups)
+    assert init_err < 1e-8
+    assert worst < 1e-8
+    assert st["flags"] == 0
+    assert st["max_err"] < 1e-8
+
+
+def test_split_flip_batched_equals_single_group():
+    """Uneven chains, 2 workgroups per chain: bitwise equal to one workgroup per chain."""
+    _say("[split batched]")
+    cases = [(8, 16, 2.0, 1), (6, 20, 2.0, 2), (7, 8, 1.0, 3), (10, 12, 1.5, 4)]
+    specs = []
+    for side, L, beta, seed in cases:
+        k, f = square_case(side, L, beta, 4.0, seed)
+        specs.append(spec_from(k, beta, L, 3.0 + seed, f, seed=seed))
+    one = SweepEngine(specs, mode="dqmc", delay=16, n_stab=5, rng="philox", flip_groups=1)
+    two = SweepEngine(specs, mode="dqmc", delay=16, n_stab=5, rng="philox", flip_groups=2)
+    for e in (one, two):
+        e.sweep(3, measure=True)
+        e.sync()
+    for i in range(len(specs)):
+        assert np.array_equal(one.green(i, which=1), two.green(i, which=1)), i
+        assert np.array_equal(one.green(i, which=0), two.green(i, which=0)), i
+        assert np.array_equal(one.field(i), two.field(i)), i
+        assert two.stats(i)["flags"] == 0
+    one.close()
+    two.close()
 
 
 def test_dqmc_u0_closed_form():
