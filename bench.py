@@ -262,7 +262,7 @@ def main():
         if os.path.exists(tpath):
             with open(tpath) as f:
                 tr = json.load(f)
-            if tr.get("delay") in (None, args.delay):
+            if tr.get("delay") in (None, out["config"]["delay"]):
                 out["roofline"]["traffic"] = tr["bytes_per_launch"]
                 out["roofline"]["traffic_source"] = os.path.relpath(tpath, ROOT)
     if not args.no_cpu:
