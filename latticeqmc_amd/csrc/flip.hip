@@ -351,10 +351,14 @@ __device__ void form_z(const T* G, size_t n2, int N, int Np, int b0, int kb, int
   }
 }
 
-// Phase 2 for one 16-row strip rs (spin rs / nct, rows (rs % nct) * 16 ..) of G, by one wavefront.
+// Phase 2 for one 16-row strip rs (spin rs / nct, rows (rs % nct) * 16 ..) of G, by one
+// wavefront: the column-tile groups cg, cg + ng, ... of RU tiles each (default: all).  P is
+// formed from G0[strip rows, B], or from the 16 x 16 copy pin of it when given (a strip
+// whose column groups are shared by several wavefronts: one of them may already have
+// updated the tile holding columns B).
 template <typename T, int K, int RU>
 __device__ void update_strip(T* G, size_t n2, int N, int Np, int b0, int kb, int m, const double* Mx,
-                             const T* zws, int rs) {
+                             const T* zws, int rs, int cg = 0, int ng = 1, const double* pin = nullptr) {
   using acc_t = typename MfmaOp<T>::acc_t;
   constexpr int JT = (K + 15) / 16;
   const int lane = threadIdx.x & 63;
@@ -376,13 +380,14 @@ __device__ void update_strip(T* G, size_t n2, int N, int Np, int b0, int kb, int
       for (int kk = 0; kk < kb; kk += 4) {
         const int q = kk + lk;
         const T x = (T)ld0(Mxs, (size_t)q * K + a, q < kb && a < m);
-        const T y = ldg0(Gs, (size_t)r * N + b0 + q, r < N && q < kb);
+        const T y = pin ? (T)ld0(pin, (size_t)li * 16 + q, r < N && q < kb)
+                        : ldg0(Gs, (size_t)r * N + b0 + q, r < N && q < kb);
         pt[jt] = MfmaOp<T>::mma(x, y, pt[jt]);
       }
       pt[jt] = -pt[jt];
     }
   }
-  for (int c0 = 0; c0 < nct; c0 += RU) {
+  for (int c0 = cg * RU; c0 < nct; c0 += RU * ng) {
     acc_t acc[RU];
     T zv[RU][JT][4];
 #pragma unroll
@@ -523,7 +528,8 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
   int* sb = aidx + K;                          // [K] field entries of the block
   int* m_sh = sb + K;                          // [4] accept count of the block
   double* Mxp = reinterpret_cast<double*>(m_sh + 4);  // [2][K][K] Mx of the previous block (W > 1)
-  int8_t* fl = reinterpret_cast<int8_t*>(Mxp + 2 * K * K);  // [N] the slice's field (flipped here)
+  double* Pin = Mxp + 2 * K * K;  // [2][16][16] G0[rows, columns] of the deferred strips (K = 16)
+  int8_t* fl = reinterpret_cast<int8_t*>(Pin + (K == 16 ? 512 : 0));  // [N] the slice's field
 
   const unsigned long long sweep = a.st[c].sweep;
   unsigned long long nacc = 0;
@@ -538,16 +544,34 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
   // The deferred strips of the previous block (its own rows, after the chain barrier), with
   // that block's Mx kept in Mxp and its Z tiles still in the workspace.  Each is applied by
   // its owner workgroup; while wave 0 runs the next serial phase the strips go to waves 1 ..
+  // Each strip's column-tile groups are shared out over those waves, so a strip is not
+  // left to one wavefront's latency.
+  // For K = 16 (one strip per spin) each strip's column-tile groups are shared out over
+  // those waves, with P formed from the copy Pin; for K = 32 a wave takes whole strips.
   auto deferred_strips = [&](bool serial_busy) {
+    constexpr int RUd = K >= 16 ? 4 : 2;
     const int s0 = pb0 >> 4, s1 = (pb0 + pkb - 1) >> 4;
-    int i = 0;
-    for (int sp = 0; sp < 2; ++sp)
-      for (int st = s0; st <= s1; ++st, ++i) {
-        const int rs = sp * nct + st, own = rs % tw;
-        if (own / nw != w) continue;
-        const int dw = serial_busy ? 1 + i % (nw - 1) : own % nw;
-        if (wave == dw) update_strip<T, K, (K >= 16 ? 4 : 2)>(G, n2, N, Np, pb0, pkb, pm, Mxp, wsT, rs);
+    const int w0 = serial_busy ? 1 : 0, nwv = nw - w0;
+    if constexpr (K == 16) {
+      const int ngr = (nct + RUd - 1) / RUd;  // column-tile groups per strip
+      int item = 0;                           // (strip, group) items of this workgroup
+      for (int sp = 0; sp < 2; ++sp) {
+        const int rs = sp * nct + s0;
+        if ((rs % tw) / nw != w) continue;
+        for (int g = 0; g < ngr; ++g, ++item)
+          if (wave == w0 + item % nwv)
+            update_strip<T, K, RUd>(G, n2, N, Np, pb0, pkb, pm, Mxp, wsT, rs, g, ngr, Pin + sp * 256);
       }
+    } else {
+      int i = 0;
+      for (int sp = 0; sp < 2; ++sp)
+        for (int st = s0; st <= s1; ++st, ++i) {
+          const int rs = sp * nct + st, own = rs % tw;
+          if (own / nw != w) continue;
+          const int dw = serial_busy ? 1 + i % (nw - 1) : own % nw;
+          if (wave == dw) update_strip<T, K, RUd>(G, n2, N, Np, pb0, pkb, pm, Mxp, wsT, rs);
+        }
+    }
   };
 
   for (int b0 = 0; b0 < N; b0 += K) {
@@ -623,10 +647,9 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
           const double cy = cfs * yq;
 #pragma unroll
           for (int i = 0; i < R; ++i) col[i] = fma(-cy, xp[i], col[i]);
-          if (lane == src) {
+          if (lane == src) {  // rows >= kb of the block parts are never read
 #pragma unroll
-            for (int i = 0; i < R; ++i)
-              if (h * R + i < kb) Xsb[sp * K * K + (h * R + i) * K + m] = cfs * xp[i];
+            for (int i = 0; i < R; ++i) Xsb[sp * K * K + (h * R + i) * K + m] = cfs * xp[i];
           }
           if (live && h == 0) Ysb[sp * K * K + q * K + m] = yq;
           if (lane == 0) {
@@ -699,7 +722,21 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
     const int isp = wave >> 1;
     const bool lower = (wave & 1) != 0;
     const int j = lane;
-    if (wave < 4) {
+    if (wave < 4 && K <= 16) {
+      // fully unrolled: row r sums only its entries p > r (the table is strictly upper
+      // triangular with exact zeros elsewhere, so the result is the same)
+      const double* Tw = Xsb + isp * 2 * K * K + (lower ? K * K : 0);
+#pragma unroll
+      for (int p = 0; p < K; ++p) inv[p] = p == j ? 1.0 : 0.0;
+#pragma unroll
+      for (int r = K - 2; r >= 0; --r) {
+        double v4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int p = r + 1; p < K; ++p) v4[p & 3] -= Tw[r * K + p] * inv[p];
+        const double v = (v4[0] + v4[1]) + (v4[2] + v4[3]);
+        if (r < j) inv[r] = v;
+      }
+    } else if (wave < 4) {
       const double* Tw = Xsb + isp * 2 * K * K + (lower ? K * K : 0);
 #pragma unroll
       for (int p = 0; p < K; ++p) inv[p] = p == j ? 1.0 : 0.0;
@@ -753,6 +790,14 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
       pkb = kb;
       pm = m;
       for (int e = threadIdx.x; e < 2 * K * K; e += blockDim.x) Mxp[e] = Xsb[e];
+      if constexpr (K == 16) {
+        // rows B x columns B of both spins (untouched until the deferred update)
+        for (int e = threadIdx.x; e < 512; e += blockDim.x) {
+          const int sp = e >> 8, i = (e >> 4) & 15, q = e & 15;
+          const bool ok = b0 + i < N && q < kb;
+          Pin[e] = ok ? (double)G[sp * n2 + (size_t)(b0 + i) * N + b0 + q] : 0.0;
+        }
+      }
     }
     __syncthreads();
   }
@@ -987,7 +1032,8 @@ int flip_delayed_threads_limit(int k) { return flip_delayed_max_threads(k); }
 
 size_t flip_delayed_shmem(int nmax, int k, int threads, int fp32) {
   (void)fp32;
-  return sizeof(double) * (threads + 6 * (size_t)k * k + 3 * (size_t)k) + sizeof(int) * (2 * k + 4) + (size_t)nmax;
+  return sizeof(double) * (threads + 6 * (size_t)k * k + 3 * (size_t)k + (k == 16 ? 512 : 0)) +
+         sizeof(int) * (2 * k + 4) + (size_t)nmax;
 }
 
 // Bytes of the per-chain delayed-update workspace (the Z tiles of both spins, one set per
