@@ -45,62 +45,53 @@ __device__ inline double scale_val(int kind, const double* s, int idx) {
 
 constexpr int BM = 64, BN = 64, BK = 16, PAD = 4;
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks, int ntasks) {
+// One 64x64 tile of C = alpha diag(rs) op(A) diag(ks) op(B) diag(cs) + beta C for one
+// transpose combination (TA, TB known at compile time, so every operand load is one load
+// through a computed index).  The k-scaling factors of each k-tile are formed once, by 16
+// threads, into LDS and applied as the A tile is stored; the row and column factors of
+// the epilogue are formed once per lane.
+template <typename T, int TA, int TB>
+__device__ void gemm_tile(const GemmTask& tk, int m0, int n0, T (*As)[BM + PAD], T (*Bs)[BN + PAD],
+                          double* ksv) {
   using acc_t = typename Mfma<T>::acc_t;
-  const int tile = blockIdx.x;
-  int lo = 0, hi = ntasks - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (tasks[mid].tile_base <= tile) lo = mid; else hi = mid - 1;
-  }
-  const GemmTask tk = tasks[lo];
-  const int local = tile - tk.tile_base;
-  const int m0 = (local / tk.tiles_n) * BM;
-  const int n0 = (local % tk.tiles_n) * BN;
   const int M = tk.M, N = tk.N, K = tk.K;
   const T* __restrict__ A = static_cast<const T*>(tk.A);
   const T* __restrict__ B = static_cast<const T*>(tk.B);
   T* __restrict__ C = static_cast<T*>(tk.C);
-
-  __shared__ T As[BK][BM + PAD];
-  __shared__ T Bs[BK][BN + PAD];
-
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
+  const bool kscale = tk.ks_kind != SC_NONE;
 
   // global->register staging coordinates
   // A: non-trans: row m = tid/4, k = (tid%4)*4 + q ; trans: k = tid/16, m = (tid%16)*4 + q
   // B: non-trans: k = tid/16, n = (tid%16)*4 + q ; trans: n = tid/4, k = (tid%4)*4 + q
   T ra[4], rb[4];
+  double kf = 1.0;  // threads < 16: the k-scaling factor of row k0 + tid of the next tile
   auto load_tiles = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      int m, k;
-      if (!tk.transA) { m = m0 + (tid >> 2); k = k0 + ((tid & 3) << 2) + q; }
-      else { k = k0 + (tid >> 4); m = m0 + ((tid & 15) << 2) + q; }
-      T v = T(0);
-      if (m < M && k < K) {
-        v = tk.transA ? A[(size_t)k * tk.lda + m] : A[(size_t)m * tk.lda + k];
-        if (tk.ks_kind) v = v * T(scale_val(tk.ks_kind, tk.ks, k));
-      }
-      ra[q] = v;
-      int n, kb;
-      if (!tk.transB) { kb = k0 + (tid >> 4); n = n0 + ((tid & 15) << 2) + q; }
-      else { n = n0 + (tid >> 2); kb = k0 + ((tid & 3) << 2) + q; }
-      T w = T(0);
-      if (n < N && kb < K) w = tk.transB ? B[(size_t)n * tk.ldb + kb] : B[(size_t)kb * tk.ldb + n];
-      rb[q] = w;
+      const int m = TA ? m0 + ((tid & 15) << 2) + q : m0 + (tid >> 2);
+      const int k = TA ? k0 + (tid >> 4) : k0 + ((tid & 3) << 2) + q;
+      const size_t ia = TA ? (size_t)k * tk.lda + m : (size_t)m * tk.lda + k;
+      ra[q] = (m < M && k < K) ? A[ia] : T(0);
+      const int n = TB ? n0 + (tid >> 2) : n0 + ((tid & 15) << 2) + q;
+      const int kb = TB ? k0 + ((tid & 3) << 2) + q : k0 + (tid >> 4);
+      const size_t ib = TB ? (size_t)n * tk.ldb + kb : (size_t)kb * tk.ldb + n;
+      rb[q] = (n < N && kb < K) ? B[ib] : T(0);
     }
+    if (kscale && tid < BK) kf = k0 + tid < K ? scale_val(tk.ks_kind, tk.ks, k0 + tid) : 1.0;
   };
   auto store_tiles = [&]() {
+    // ksv was written before the barrier that precedes this call
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      if (!tk.transA) As[((tid & 3) << 2) + q][tid >> 2] = ra[q];
-      else As[tid >> 4][((tid & 15) << 2) + q] = ra[q];
-      if (!tk.transB) Bs[tid >> 4][((tid & 15) << 2) + q] = rb[q];
+      const int ka = TA ? (tid >> 4) : ((tid & 3) << 2) + q;
+      const T av = kscale ? ra[q] * T(ksv[ka]) : ra[q];
+      if (!TA) As[((tid & 3) << 2) + q][tid >> 2] = av;
+      else As[tid >> 4][((tid & 15) << 2) + q] = av;
+      if (!TB) Bs[tid >> 4][((tid & 15) << 2) + q] = rb[q];
       else Bs[((tid & 3) << 2) + q][tid >> 2] = rb[q];
     }
   };
@@ -113,6 +104,8 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
 
   const int nk = (K + BK - 1) / BK;
   load_tiles(0);
+  if (kscale && tid < BK) ksv[tid] = kf;
+  __syncthreads();
   store_tiles();
   __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
@@ -129,7 +122,9 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
       acc[1][0] = Mfma<T>::mma(a1, b0, acc[1][0]);
       acc[1][1] = Mfma<T>::mma(a1, b1, acc[1][1]);
     }
-    __syncthreads();
+    // ksv was last read by the store phase before this tile's products (a barrier apart)
+    if (kt + 1 < nk && kscale && tid < BK) ksv[tid] = kf;
+    __syncthreads();  // every read of As and Bs for this tile is done; ksv is visible
     if (kt + 1 < nk) {
       store_tiles();
       __syncthreads();
@@ -137,24 +132,63 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
   }
 
   // epilogue: lane holds rows Mfma<T>::row(lane, r), column lane%16 of each 16x16 block
+  const bool plain = tk.rs_kind == SC_NONE && tk.cs_kind == SC_NONE && tk.alpha == 1.0 && tk.beta == 0.0;
+  double rsv[2][4], csv[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+    csv[j] = plain || col >= N ? 1.0 : scale_val(tk.cs_kind, tk.cs, col);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = m0 + wm * 32 + i * 16 + Mfma<T>::row(lane, r);
+      rsv[i][r] = plain || row >= M ? 1.0 : tk.alpha * scale_val(tk.rs_kind, tk.rs, row);
+    }
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int col = n0 + wn * 32 + j * 16 + (lane & 15);
       if (col >= N) continue;
-      const double csv = scale_val(tk.cs_kind, tk.cs, col);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm * 32 + i * 16 + Mfma<T>::row(lane, r);
         if (row >= M) continue;
-        double v = (double)acc[i][j][r];
-        v = tk.alpha * scale_val(tk.rs_kind, tk.rs, row) * v * csv;
         T* p = C + (size_t)row * tk.ldc + col;
-        if (tk.beta != 0.0) v += tk.beta * (double)(*p);
-        *p = (T)v;
+        if (plain) {
+          *p = acc[i][j][r];
+        } else {
+          double v = rsv[i][r] * (double)acc[i][j][r] * csv[j];
+          if (tk.beta != 0.0) v += tk.beta * (double)(*p);
+          *p = (T)v;
+        }
       }
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks, int ntasks) {
+  const int tile = blockIdx.x;
+  int lo = 0, hi = ntasks - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].tile_base <= tile) lo = mid; else hi = mid - 1;
+  }
+  const GemmTask tk = tasks[lo];
+  const int local = tile - tk.tile_base;
+  const int m0 = (local / tk.tiles_n) * BM;
+  const int n0 = (local % tk.tiles_n) * BN;
+  __shared__ T As[BK][BM + PAD];
+  __shared__ T Bs[BK][BN + PAD];
+  __shared__ double ksv[BK];
+  switch (tk.transA * 2 + tk.transB) {
+    case 0: gemm_tile<T, 0, 0>(tk, m0, n0, As, Bs, ksv); break;
+    case 1: gemm_tile<T, 0, 1>(tk, m0, n0, As, Bs, ksv); break;
+    case 2: gemm_tile<T, 1, 0>(tk, m0, n0, As, Bs, ksv); break;
+    default: gemm_tile<T, 1, 1>(tk, m0, n0, As, Bs, ksv); break;
+  }
 }
 
 void launch_gemm(int fp32, const GemmTask* d_tasks, int ntasks, int ntiles, hipStream_t s) {
