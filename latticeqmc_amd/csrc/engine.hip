@@ -27,7 +27,8 @@
 #include "common.h"
 
 namespace lqmc {
-void launch_gemm(int fp32, const GemmTask* d_tasks, int ntasks, int ntiles, hipStream_t s);
+void launch_gemm(int fp32, int cls, const GemmTask* d_tasks, int ntasks, int ntiles, hipStream_t s);
+int gemm_class(const GemmTask& t);
 int gemm_tiles(int M, int N, int* tiles_n);
 void launch_qr_blocked(const QrTask* d, int n, int nmax, ChainState* st, hipStream_t s);
 void launch_trsm(const TrsmTask* d, int n, int nmax, ChainState* st, hipStream_t s);
@@ -232,20 +233,27 @@ struct Builder {
     return off;
   }
 
+  // One launch per kernel class (transpose case, full tiles) present among the tasks; the
+  // tasks of one call are independent, so the split launches may run in any order.
   void gemm(std::vector<Op>& ops, std::vector<GemmTask>& tasks, int fp32) {
     if (tasks.empty()) return;
-    int base = 0;
-    double flops = 0;
-    for (auto& t : tasks) {
-      int tn;
-      int nt = gemm_tiles(t.M, t.N, &tn);
-      t.tiles_n = tn;
-      t.tile_base = base;
-      base += nt;
-      flops += 2.0 * t.M * t.N * t.K;
+    for (int cls = 0; cls < 8; ++cls) {
+      std::vector<GemmTask> part;
+      for (auto& t : tasks)
+        if (gemm_class(t) == cls) part.push_back(t);
+      if (part.empty()) continue;
+      int base = 0;
+      double flops = 0;
+      for (auto& t : part) {
+        int tn;
+        int nt = gemm_tiles(t.M, t.N, &tn);
+        t.tiles_n = tn;
+        t.tile_base = base;
+        base += nt;
+        flops += 2.0 * t.M * t.N * t.K;
+      }
+      ops.push_back(Op{OP_GEMM, fp32, push(part), (int)part.size(), base, cls, 0, flops});
     }
-    Op op{OP_GEMM, fp32, push(tasks), (int)tasks.size(), base, 0, 0, flops};
-    ops.push_back(op);
     tasks.clear();
   }
   void qr(std::vector<Op>& ops, std::vector<QrTask>& tasks) {
@@ -578,7 +586,7 @@ static void run_ops(lqmc_ctx* x, const std::vector<Op>& ops, int measure, hipStr
     }
     const char* d = x->d_desc + op.off;
     switch (op.kind) {
-      case OP_GEMM: launch_gemm(op.fp32, reinterpret_cast<const GemmTask*>(d), op.count, op.ntiles, s); break;
+      case OP_GEMM: launch_gemm(op.fp32, op.nmax, reinterpret_cast<const GemmTask*>(d), op.count, op.ntiles, s); break;
       case OP_QR: launch_qr_blocked(reinterpret_cast<const QrTask*>(d), op.count, op.nmax, x->d_state, s); break;
       case OP_TRSM: launch_trsm(reinterpret_cast<const TrsmTask*>(d), op.count, op.nmax, x->d_state, s); break;
       case OP_SOLVE: launch_solve(reinterpret_cast<const SolveTask*>(d), op.count, op.nmax, x->d_state, s); break;

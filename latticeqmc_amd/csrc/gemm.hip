@@ -50,7 +50,9 @@ constexpr int BM = 64, BN = 64, BK = 16, PAD = 4;
 // through a computed index).  The k-scaling factors of each k-tile are formed once, by 16
 // threads, into LDS and applied as the A tile is stored; the row and column factors of
 // the epilogue are formed once per lane.
-template <typename T, int TA, int TB>
+//  FULL: every tile of the task lies inside C and K is a multiple of BK, so the operand
+//  loads need no bounds checks.
+template <typename T, int TA, int TB, bool FULL>
 __device__ void gemm_tile(const GemmTask& tk, int m0, int n0, T (*As)[BM + PAD], T (*Bs)[BN + PAD],
                           double* ksv) {
   using acc_t = typename Mfma<T>::acc_t;
@@ -75,11 +77,11 @@ __device__ void gemm_tile(const GemmTask& tk, int m0, int n0, T (*As)[BM + PAD],
       const int m = TA ? m0 + ((tid & 15) << 2) + q : m0 + (tid >> 2);
       const int k = TA ? k0 + (tid >> 4) : k0 + ((tid & 3) << 2) + q;
       const size_t ia = TA ? (size_t)k * tk.lda + m : (size_t)m * tk.lda + k;
-      ra[q] = (m < M && k < K) ? A[ia] : T(0);
+      ra[q] = (FULL || (m < M && k < K)) ? A[ia] : T(0);
       const int n = TB ? n0 + (tid >> 2) : n0 + ((tid & 15) << 2) + q;
       const int kb = TB ? k0 + ((tid & 3) << 2) + q : k0 + (tid >> 4);
       const size_t ib = TB ? (size_t)n * tk.ldb + kb : (size_t)kb * tk.ldb + n;
-      rb[q] = (n < N && kb < K) ? B[ib] : T(0);
+      rb[q] = (FULL || (n < N && kb < K)) ? B[ib] : T(0);
     }
     if (kscale && tid < BK) kf = k0 + tid < K ? scale_val(tk.ks_kind, tk.ks, k0 + tid) : 1.0;
   };
@@ -168,7 +170,9 @@ __device__ void gemm_tile(const GemmTask& tk, int m0, int n0, T (*As)[BM + PAD],
     }
 }
 
-template <typename T>
+// One kernel per (transpose case, full tiles) class: a launch holds tasks of one class
+// only, so each kernel keeps the registers of a single tile body.
+template <typename T, int TA, int TB, bool FULL>
 __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks, int ntasks) {
   const int tile = blockIdx.x;
   int lo = 0, hi = ntasks - 1;
@@ -183,20 +187,36 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
   __shared__ T As[BK][BM + PAD];
   __shared__ T Bs[BK][BN + PAD];
   __shared__ double ksv[BK];
-  switch (tk.transA * 2 + tk.transB) {
-    case 0: gemm_tile<T, 0, 0>(tk, m0, n0, As, Bs, ksv); break;
-    case 1: gemm_tile<T, 0, 1>(tk, m0, n0, As, Bs, ksv); break;
-    case 2: gemm_tile<T, 1, 0>(tk, m0, n0, As, Bs, ksv); break;
-    default: gemm_tile<T, 1, 1>(tk, m0, n0, As, Bs, ksv); break;
+  gemm_tile<T, TA, TB, FULL>(tk, m0, n0, As, Bs, ksv);
+}
+
+int gemm_class(const GemmTask& t) {
+  const int full = t.M % BM == 0 && t.N % BN == 0 && t.K % BK == 0;
+  return (t.transA ? 4 : 0) + (t.transB ? 2 : 0) + full;
+}
+
+template <typename T>
+static void launch_gemm_t(int cls, const GemmTask* d_tasks, int ntasks, int ntiles, hipStream_t s) {
+  switch (cls) {
+#define LQMC_GEMM_CASE(C, TA, TB, F) \
+    case C: hipLaunchKernelGGL((k_gemm<T, TA, TB, F>), dim3(ntiles), dim3(256), 0, s, d_tasks, ntasks); break;
+    LQMC_GEMM_CASE(0, 0, 0, false)
+    LQMC_GEMM_CASE(1, 0, 0, true)
+    LQMC_GEMM_CASE(2, 0, 1, false)
+    LQMC_GEMM_CASE(3, 0, 1, true)
+    LQMC_GEMM_CASE(4, 1, 0, false)
+    LQMC_GEMM_CASE(5, 1, 0, true)
+    LQMC_GEMM_CASE(6, 1, 1, false)
+    LQMC_GEMM_CASE(7, 1, 1, true)
+#undef LQMC_GEMM_CASE
+    default: break;
   }
 }
 
-void launch_gemm(int fp32, const GemmTask* d_tasks, int ntasks, int ntiles, hipStream_t s) {
+void launch_gemm(int fp32, int cls, const GemmTask* d_tasks, int ntasks, int ntiles, hipStream_t s) {
   if (ntasks <= 0 || ntiles <= 0) return;
-  if (fp32)
-    hipLaunchKernelGGL(k_gemm<float>, dim3(ntiles), dim3(256), 0, s, d_tasks, ntasks);
-  else
-    hipLaunchKernelGGL(k_gemm<double>, dim3(ntiles), dim3(256), 0, s, d_tasks, ntasks);
+  if (fp32) launch_gemm_t<float>(cls, d_tasks, ntasks, ntiles, s);
+  else launch_gemm_t<double>(cls, d_tasks, ntasks, ntiles, s);
 }
 
 int gemm_tiles(int M, int N, int* tiles_n) {
