@@ -73,3 +73,60 @@ def local_moment_from_docc(density, docc):
     (the sweep-averaged estimator; the reference's local_moment, tools.py:87-109, uses
     products of averaged fillings instead)."""
     return np.asarray(density) - 2.0 * np.asarray(docc)
+
+
+# ---- free-fermion Green's functions (the reference's GF tools, tools.py:127-200) ----------------
+
+def matsubara_frequencies(points, beta):
+    """Fermionic Matsubara frequencies i w_n = i pi (2 n + 1) / beta (tools.py:127-142)."""
+    n = np.asarray(points)
+    if not np.issubdtype(n.dtype, np.integer):
+        raise TypeError("Matsubara indices must be integers")
+    return 1j * np.pi * (2 * n + 1) / beta
+
+
+def fermi_fct(eps, beta):
+    """Fermi function 1 / (exp(beta eps) + 1), written as (1 - tanh(beta eps / 2)) / 2 so that
+    large |beta eps| neither overflows nor loses the small tail (tools.py:145-167)."""
+    return 0.5 * (1.0 - np.tanh(0.5 * beta * np.asarray(eps)))
+
+
+def decompose(a):
+    """Eigen-decomposition of a Hermitian matrix as (eigenvectors, eigenvalues, inverse
+    eigenvectors) (tools.py:170-172)."""
+    xi, rv = np.linalg.eigh(a)
+    return rv, xi, np.linalg.inv(rv)
+
+
+def reconstruct(rv, xi, rv_inv, diag=False):
+    """rv diag(xi) rv_inv over the trailing axes of xi, or only its diagonal (tools.py:175-179)."""
+    xi = np.asarray(xi)
+    if diag:
+        return np.einsum("ij,...j,ji->...i", rv, xi, rv_inv)
+    return (rv * xi[..., np.newaxis, :]) @ rv_inv
+
+
+def pole_gf_tau(tau, poles, weights, beta):
+    """Imaginary-time Green's function of a sum of poles, 0 <= tau <= beta (tools.py:182-190):
+    G(tau) = -sum_p w_p exp(-tau e_p) (1 - f(e_p)), evaluated as exp((beta - tau) e_p) f(e_p)
+    for negative poles so that neither branch overflows."""
+    poles, weights = np.atleast_1d(*np.broadcast_arrays(poles, weights))
+    tau = np.asarray(tau, dtype=np.float64)
+    if np.any((tau < 0.0) | (tau > beta)):
+        raise ValueError("tau must lie in [0, beta]")
+    t = tau.reshape(tau.shape + (1,) * poles.ndim)
+    pos = poles.real >= 0
+    expo = np.where(pos, -t * poles, (beta - t) * poles)
+    occ = np.where(pos, fermi_fct(-poles, beta), fermi_fct(poles, beta))
+    return -np.sum(weights * np.exp(expo) * occ, axis=-1)
+
+
+def compute_pole_gf_tau(ham, beta, n_tau=2049):
+    """Free Green's function of the one-body Hamiltonian ``ham`` on n_tau points of [0, beta];
+    returns (tau, G) with G of shape (site, site, tau) (tools.py:193-200).  At tau = 0+ it is
+    -(I + exp(-beta ham))^-1, minus the equal-time G that the sweep measures at U = 0."""
+    rv, xi, rv_inv = decompose(ham)
+    tau = np.linspace(0.0, beta, num=n_tau)
+    diag = pole_gf_tau(tau, xi[..., np.newaxis], weights=1.0, beta=beta)
+    gf = reconstruct(rv, diag, rv_inv)
+    return tau, np.moveaxis(gf, 0, -1)

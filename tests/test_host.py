@@ -85,3 +85,29 @@ def test_det_mode_rejects_fp32_and_delay_before_device_use():
     for kw in ({"precision": "fp32"}, {"delay": 4}):
         with pytest.raises(LqmcError, match="determinant mode"):
             SweepEngine([spec], mode="det", rng="replay", **kw)
+
+
+def test_free_gf_tools_against_closed_forms():
+    """compute_pole_gf_tau at tau = 0 and beta against (I + e^{-beta K})^-1 (the U = 0 equal-time
+    G), reconstruct(decompose(K)) = K, the Fermi function's symmetry and the Matsubara grid."""
+    from latticeqmc_amd import tools
+    from oracle import exact, model
+    k = model.square_k(3, 3) - 0.7 * np.eye(9)
+    beta = 4.0
+    tau, g = tools.compute_pole_gf_tau(k, beta, n_tau=33)
+    g_eq = exact.free_g(k, beta)
+    assert np.abs(g[:, :, 0] + g_eq).max() < 1e-12
+    assert np.abs(g[:, :, -1] - (g_eq - np.eye(9))).max() < 1e-12  # -f(K) = G - I
+    # a single pole: G(tau) = -exp(-tau e) (1 - f(e))
+    e = 1.3
+    t = np.array([0.0, 0.5, 2.0])
+    want = -np.exp(-t * e) / (1.0 + np.exp(-beta * e))
+    assert np.allclose(tools.pole_gf_tau(t, e, 1.0, beta), want, rtol=1e-13, atol=0)
+    rv, xi, rvi = tools.decompose(k)
+    assert np.abs(tools.reconstruct(rv, xi, rvi) - k).max() < 1e-12
+    assert np.abs(tools.reconstruct(rv, xi, rvi, diag=True) - np.diag(k)).max() < 1e-12
+    x = np.linspace(-3, 3, 7)
+    assert np.allclose(tools.fermi_fct(x, 2.0) + tools.fermi_fct(-x, 2.0), 1.0)
+    assert np.allclose(tools.matsubara_frequencies(np.arange(3), 2.0), 1j * np.pi * np.array([1, 3, 5]) / 2.0)
+    with pytest.raises(ValueError):
+        tools.pole_gf_tau(-0.1, e, 1.0, beta)
