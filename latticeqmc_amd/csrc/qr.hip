@@ -37,15 +37,26 @@ __device__ inline qr_v4d qr_mma(double a, double b, qr_v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Keeps the instruction scheduler from moving code across this point: the loads issued
+// before it are all in flight before the arithmetic after it starts (left to itself the
+// scheduler interleaves each LDS load with its use and waits on every one).
+__device__ inline void qr_sched_fence() { __builtin_amdgcn_sched_barrier(0); }
+
 // X[r0:N, c_begin:c_end] <- (I - V T' V^T) X[r0:N, c_begin:c_end] on the FP64 matrix
 // cores, V = pnl (explicit reflectors, rows r0..N-1, w <= 16 columns, row stride ps),
 // T' = T^T (trans) or T (w x w, leading dimension nb).  One wavefront per 16-column tile
 // (NW wavefronts in the workgroup): W = V^T X (K = rows), W' = T' W, X -= V W'.  In the
 // 16x16x4 FP64 layout the accumulator element v of lane l is row l/16 + 4v, column
 // l%16 -- exactly the B operand lane l supplies for k-step v -- so W and W' chain
-// through registers.  Loads are unconditional (clamped), so a loop's loads overlap.
-// Ends with a workgroup barrier.
-template <int NW>
+// through registers.  A tile is one wavefront's serial walk down the rows, so its loads
+// are unconditional (clamped: zero beyond row N, which adds nothing to the products) and
+// issued in batches ahead of the products -- KB k-steps of V^T X, RB 16-row tiles of the
+// update -- to keep many global loads in flight instead of waiting on each.  That pays
+// where a few wavefronts per workgroup share a compute unit with other workgroups (Q
+// formation: KB = 8, RB = 4); with 16 wavefronts of one factorisation all streaming tiles
+// the compute unit's memory path is the limit and the larger batches only add register
+// pressure (KB = RB = 1).  Ends with a workgroup barrier.
+template <int NW, int KB, int RB>
 __device__ void qr_apply_block(double* X, int N, int r0, int c_begin, int c_end, const double* pnl, int ps,
                                int w, const double* Tm, int nb, bool trans) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -56,11 +67,17 @@ __device__ void qr_apply_block(double* X, int N, int r0, int c_begin, int c_end,
     const int c = c_begin + ct * 16 + li;
     const bool cok = c < c_end;
     qr_v4d wt = {0.0, 0.0, 0.0, 0.0};
-    for (int kk = r0; kk < N; kk += 4) {
-      const int r = kk + lk;
-      const double a = ld0(pnl, (size_t)r * ps + li, r < N && li < w);
-      const double b = ldg0(Xg, (size_t)r * N + c, r < N && cok);
-      wt = qr_mma(a, b, wt);
+    for (int kk = r0; kk < N; kk += 4 * KB) {
+      double a[KB], b[KB];
+#pragma unroll
+      for (int u = 0; u < KB; ++u) {
+        const int r = kk + 4 * u + lk;
+        a[u] = ld0(pnl, (size_t)r * ps + li, r < N && li < w);
+        b[u] = ldg0(Xg, (size_t)r * N + c, r < N && cok);
+      }
+      qr_sched_fence();
+#pragma unroll
+      for (int u = 0; u < KB; ++u) wt = qr_mma(a[u], b[u], wt);
     }
     qr_v4d w2 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -69,34 +86,38 @@ __device__ void qr_apply_block(double* X, int N, int r0, int c_begin, int c_end,
       const double a = ld0(Tm, trans ? (size_t)p * nb + li : (size_t)li * nb + p, li < w && p < w);
       w2 = qr_mma(a, wt[s], w2);
     }
-    for (int rt = r0; rt < N; rt += 16) {
-      qr_v4d acc;
+    for (int rt = r0; rt < N; rt += 16 * RB) {
+      qr_v4d acc[RB];
+      double a[RB][4];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = rt + lk + 4 * v;
-        acc[v] = ldg0(Xg, (size_t)row * N + c, row < N && cok);
+      for (int t = 0; t < RB; ++t) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = rt + 16 * t + lk + 4 * v;
+          acc[t][v] = ldg0(Xg, (size_t)row * N + c, row < N && cok);
+        }
+        const int ra = rt + 16 * t + li;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int q = 4 * s + lk;
+          a[t][s] = -ld0(pnl, (size_t)ra * ps + q, ra < N && q < w);
+        }
       }
-      const int ra = rt + li;
+      qr_sched_fence();
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int q = 4 * s + lk;
-        const double a = -ld0(pnl, (size_t)ra * ps + q, ra < N && q < w);
-        acc = qr_mma(a, w2[s], acc);
-      }
+      for (int t = 0; t < RB; ++t) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = rt + lk + 4 * v;
-        if (row < N && cok) Xg[(size_t)row * N + c] = acc[v];
+        for (int s = 0; s < 4; ++s) acc[t] = qr_mma(a[t][s], w2[s], acc[t]);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = rt + 16 * t + lk + 4 * v;
+          if (row < N && cok) Xg[(size_t)row * N + c] = acc[t][v];
+        }
       }
     }
   }
   __syncthreads();
 }
-
-// Keeps the instruction scheduler from moving code across this point: the loads issued
-// before it are all in flight before the arithmetic after it starts (left to itself the
-// scheduler interleaves each LDS load with its use and waits on every one).
-__device__ inline void qr_sched_fence() { __builtin_amdgcn_sched_barrier(0); }
 
 // Orders LDS traffic between the lanes of one wavefront.
 __device__ inline void qr_wave_sync() {
@@ -266,15 +287,15 @@ __device__ void qr_panel_wave(double* pnl, int N, int j0, int w, int* perm, int*
 }
 
 // T (w x w, leading dimension nb) of the compact-WY form I - V T V^T = H_0 ... H_{w-1}
-// of a panel's reflectors (V explicit in pnl, rows r0..N-1, row stride ps).  Wavefront 0 forms
+// of a panel's reflectors (V explicit in pnl, rows r0..N-1, row stride ps).  The wavefront forms
 // S = V^T V on the matrix cores (A and B operands coincide), then lane p computes row p
 // of T in registers from the recurrence T[p][p] = tau_p,
 // T[p][i] = -tau_i * sum_{p <= q < i} T[p][q] S[q][i]  (rows are independent).
-// S is a 16 x 16 LDS scratch.  Ends with a workgroup barrier.
+// S is a 16 x 16 LDS scratch.  Called by one whole wavefront; ends with a wavefront sync.
 __device__ void qr_build_t(const double* pnl, int ps, int nb, int r0, int N, int w, const double* tau,
                            double* Tm, double* S) {
-  if (threadIdx.x < 64) {
-    const int lane = threadIdx.x, li = lane & 15, lk = lane >> 4;
+  {
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
     qr_v4d acc = {0.0, 0.0, 0.0, 0.0};
     for (int kk = r0; kk < N; kk += 4) {
       const int r = kk + lk;
@@ -300,7 +321,7 @@ __device__ void qr_build_t(const double* pnl, int ps, int nb, int r0, int N, int
         if (i < w) Tm[p * nb + i] = t[i];
     }
   }
-  __syncthreads();
+  qr_wave_sync();
 }
 
 __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restrict__ tasks, ChainState* st) {
@@ -408,13 +429,14 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
       else if (rr == q) pnl[(j0 + rr) * ps + q] = 1.0;
     }
     __syncthreads();
-    qr_build_t(pnl, ps, nb, j0, N, w, taus + j0, Tm, Ssc);
+    if (wave == 0) qr_build_t(pnl, ps, nb, j0, N, w, taus + j0, Tm, Ssc);
+    __syncthreads();
     double* tout = tk.Tt + (size_t)(j0 / nb) * nb * nb;
     for (int e = tid; e < nb * nb; e += QR_THREADS) {
       const int p = e / nb, q = e % nb;
       tout[e] = (p < w && q < w) ? Tm[e] : 0.0;
     }
-    if (j0 + w < N) qr_apply_block<QR_WAVES>(A, N, j0, j0 + w, N, pnl, ps, w, Tm, nb, true);
+    if (j0 + w < N) qr_apply_block<QR_WAVES, 1, 1>(A, N, j0, j0 + w, N, pnl, ps, w, Tm, nb, true);
   }
 
   // D = |diag R|; V'' = D^-1 R P^T over W, or R' = D^-1 R (unpermuted) with the pivot order
@@ -436,9 +458,9 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
       W[(size_t)i * N + perm[c]] = c >= i ? A[e] / norms[i] : 0.0;
     }
   }
-  if (tid == 0 && st) {
-    if (zero_pivot) atomicOr(&st[tk.chain].flags, 2u);
-    if (bad) atomicOr(&st[tk.chain].flags, 8u);
+  if (st) {
+    if (tid == 0 && zero_pivot) atomicOr(&st[tk.chain].flags, 2u);
+    if (bad) atomicOr(&st[tk.chain].flags, 8u);  // any thread whose columns had a NaN norm
   }
 }
 
@@ -475,7 +497,7 @@ __global__ __launch_bounds__(QF_THREADS) void k_qr_formq(const QrTask* __restric
     const double* tin = tk.Tt + (size_t)(j0 / nb) * nb * nb;
     for (int e = tid; e < nb * nb; e += QF_THREADS) Tm[e] = tin[e];
     __syncthreads();
-    qr_apply_block<QF_THREADS / 64>(Q, N, j0, max(c0, j0), c1, pnl, ps, w, Tm, nb, false);
+    qr_apply_block<QF_THREADS / 64, 8, 4>(Q, N, j0, max(c0, j0), c1, pnl, ps, w, Tm, nb, false);
   }
 }
 
