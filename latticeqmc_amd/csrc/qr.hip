@@ -180,44 +180,62 @@ __device__ inline void qr_wave_colsum(double (&v)[NB], double* red, int lane) {
   qr_wave_sync();
 }
 
+// Column sums over rows r0..N-1 of the panel, reduced over the wavefront into red[0..NB):
+// the squares of every column (SQ) or the products of column q with every column.
+template <int NB, bool SQ>
+__device__ inline void qr_panel_gram(const double* pnl, int N, int r0, int q, double* red, int lane) {
+  constexpr int PS = NB + 1;
+  double part[NB];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) part[c] = 0.0;
+  for (int r = r0 + lane; r < N; r += 64) {
+    const double* row = pnl + r * PS;
+    const double x = SQ ? 1.0 : row[q];
+#pragma unroll
+    for (int c0 = 0; c0 < NB; c0 += 8) {  // batches of 8 loads in flight
+      double rv[8];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) rv[c] = row[c0 + c];
+      qr_sched_fence();
+#pragma unroll
+      for (int c = 0; c < 8; ++c) part[c0 + c] += (SQ ? rv[c] : x) * rv[c];
+    }
+  }
+  qr_wave_colsum<NB>(part, red, lane);
+}
+
 // Householder factorisation of the panel (rows j0..N-1, w <= NB columns, in LDS with row
-// stride NB + 1) by one
-// wavefront, with panel-local column pivoting: at step q the remaining panel column of
-// largest norm over rows j..N-1 is swapped into place (and its pivot index with it).
-// The norms of all panel columns and the reflector's dot products with them are each
-// reduced in one exchange pass.  The column swaps are recorded in swp[] for the R
-// entries above the panel, which the caller permutes afterwards.  On return pnl holds R
-// (on/above the diagonal) and the reflectors below it.
+// stride NB + 1) by one wavefront, with panel-local column pivoting: at step q the
+// remaining panel column of largest norm over rows j..N-1 is swapped into place (and its
+// pivot index with it).  One wavefront reduction per column: y_c = x^T col_c over rows
+// j..N-1 for the pivot column x and every panel column c gives the exact |x|^2 = y_q for the
+// reflector and, with u = x - beta e_j, the products u^T col_c = y_c - beta col_c[j] of the
+// update.  The pivot choice uses the column norms downdated after each step,
+// |col_c|^2 -= R[j][c]^2 (Householder reflections keep column norms), with the norms of
+// every column recomputed when one of them has lost all but sqrt(eps) of its value since
+// its last exact computation (the safeguard of LAPACK's xLAQP2).  The column swaps are
+// recorded in swp[] for the R entries above the panel, which the caller permutes
+// afterwards.  nrm, org: NB-entry LDS scratch (current and reference squared norms); flg:
+// one int.  On return pnl holds R (on/above the diagonal) and the reflectors below it.
 template <int NB>
 __device__ void qr_panel_wave(double* pnl, int N, int j0, int w, int* perm, int* swp, double* taus,
-                              double* red, bool& zero_pivot) {
+                              double* red, double* nrm, double* org, int* flg, bool& zero_pivot) {
   const int lane = threadIdx.x & 63;
   constexpr int PS = NB + 1;  // padded row stride: lanes walking rows spread over the banks
+  constexpr double kTol = 1.4901161193847656e-08;  // sqrt(eps), eps = 2^-52
+  qr_panel_gram<NB, true>(pnl, N, j0, 0, red, lane);
+  if (lane < NB) nrm[lane] = org[lane] = red[lane];
+  qr_wave_sync();
   for (int q = 0; q < w; ++q) {
     const int j = j0 + q;
-    double part[NB];
-#pragma unroll
-    for (int c = 0; c < NB; ++c) part[c] = 0.0;
-    for (int r = j + lane; r < N; r += 64) {
-      const double* row = pnl + r * PS;
-#pragma unroll
-      for (int c0 = 0; c0 < NB; c0 += 8) {  // batches of 8 loads in flight
-        double rv[8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) rv[c] = row[c0 + c];
-        qr_sched_fence();
-#pragma unroll
-        for (int c = 0; c < 8; ++c) part[c0 + c] += rv[c] * rv[c];
-      }
-    }
-    qr_wave_colsum<NB>(part, red, lane);
     int p = q;
     double best = -1.0;
 #pragma unroll
     for (int c = 0; c < NB; ++c) {  // static trip count: the NB loads are issued together
-      const double nc = red[c];
+      const double nc = nrm[c];
       if (c >= q && c < w && nc > best) { best = nc; p = c; }
     }
+    qr_wave_sync();  // every lane has read nrm before lane 0 swaps entries
     if (p != q) {
       for (int r = j0 + lane; r < N; r += 64) {
         const double t0 = pnl[r * PS + q];
@@ -228,12 +246,17 @@ __device__ void qr_panel_wave(double* pnl, int N, int j0, int w, int* perm, int*
         const int tp = perm[j0 + q];
         perm[j0 + q] = perm[j0 + p];
         perm[j0 + p] = tp;
+        const double tn = nrm[q], to = org[q];
+        nrm[q] = nrm[p]; org[q] = org[p];
+        nrm[p] = tn; org[p] = to;
       }
     }
-    if (lane == 0) swp[q] = p;
+    if (lane == 0) { swp[q] = p; *flg = 0; }
     qr_wave_sync();
+    qr_panel_gram<NB, false>(pnl, N, j, q, red, lane);
     const double alpha = pnl[j * PS + q];
-    const double nrmv = sqrt(best > 0.0 ? best : 0.0);
+    const double xx = red[q];
+    const double nrmv = sqrt(xx > 0.0 ? xx : 0.0);
     double beta, tau, scal;
     if (nrmv == 0.0) {
       beta = 0.0; tau = 0.0; scal = 0.0;
@@ -243,30 +266,19 @@ __device__ void qr_panel_wave(double* pnl, int N, int j0, int w, int* perm, int*
       tau = (beta - alpha) / beta;
       scal = 1.0 / (alpha - beta);
     }
-    for (int r = j + 1 + lane; r < N; r += 64) pnl[r * PS + q] *= scal;
-    qr_wave_sync();
+    // dp_c = tau v^T col_c with v = u / (alpha - beta); zero outside (q, w)
     double dp[NB];
+    {
+      const double* rowj = pnl + j * PS;
+      const double ts = tau * scal;
 #pragma unroll
-    for (int c = 0; c < NB; ++c) dp[c] = 0.0;
-    for (int r = j + lane; r < N; r += 64) {
-      const double* row = pnl + r * PS;
-      const double v = r == j ? 1.0 : row[q];
-#pragma unroll
-      for (int c0 = 0; c0 < NB; c0 += 8) {
-        double rv[8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) rv[c] = row[c0 + c];
-        qr_sched_fence();
-#pragma unroll
-        for (int c = 0; c < 8; ++c) dp[c0 + c] += v * rv[c];
-      }
+      for (int c = 0; c < NB; ++c) dp[c] = (c > q && c < w) ? ts * (red[c] - beta * rowj[c]) : 0.0;
     }
-    qr_wave_colsum<NB>(dp, red, lane);
-#pragma unroll
-    for (int c = 0; c < NB; ++c) dp[c] = (c > q && c < w) ? red[c] * tau : 0.0;
+    qr_wave_sync();  // row j read by every lane before it is updated
+    // update, with the reflector below the diagonal scaled into column q
     for (int r = j + lane; r < N; r += 64) {
       double* row = pnl + r * PS;
-      const double v = r == j ? 1.0 : row[q];
+      const double v = r == j ? 1.0 : row[q] * scal;
 #pragma unroll
       for (int c0 = 0; c0 < NB; c0 += 8) {
         double rv[8];
@@ -274,7 +286,8 @@ __device__ void qr_panel_wave(double* pnl, int N, int j0, int w, int* perm, int*
         for (int c = 0; c < 8; ++c) rv[c] = row[c0 + c];
         qr_sched_fence();
 #pragma unroll
-        for (int c = 0; c < 8; ++c) row[c0 + c] = rv[c] - v * dp[c0 + c];  // dp = 0 off (q, w)
+        for (int c = 0; c < 8; ++c)
+          row[c0 + c] = (c0 + c == q && r > j) ? v : rv[c] - v * dp[c0 + c];  // dp = 0 off (q, w)
       }
     }
     qr_wave_sync();
@@ -282,7 +295,19 @@ __device__ void qr_panel_wave(double* pnl, int N, int j0, int w, int* perm, int*
       pnl[j * PS + q] = beta;
       taus[j] = tau;
     }
+    // downdate the norms of the remaining columns over rows j+1..N-1
+    if (lane > q && lane < w) {
+      const double rjc = pnl[j * PS + lane];
+      const double nn = nrm[lane] - rjc * rjc;
+      nrm[lane] = nn;
+      if (!(nn > kTol * org[lane])) *flg = 1;
+    }
     qr_wave_sync();
+    if (q + 1 < w && *flg) {  // uniform: every lane reads the same flag
+      qr_panel_gram<NB, true>(pnl, N, j + 1, 0, red, lane);
+      if (lane > q && lane < w) nrm[lane] = org[lane] = red[lane];
+      qr_wave_sync();
+    }
   }
 }
 
@@ -336,11 +361,14 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
   double* Tm = pnl + (size_t)N * ps;   // [nb][nb]
   double* Ssc = Tm + nb * nb;          // [16][16]
   double* red = Ssc + 256;             // [16]
-  double* norms = red + 16;            // [N]
+  double* pnrm = red + 16;             // [16] panel column norms (squared), current
+  double* porg = pnrm + 16;            // [16] and at their last exact computation
+  double* norms = porg + 16;           // [N]
   double* taus = norms + N;            // [N]
   int* perm = reinterpret_cast<int*>(taus + N);  // [N]
   int* swp = perm + N;                 // [16] column swaps of the current panel
   int* fp = swp + 16;                  // [16] resulting panel column order
+  int* pflg = fp + 16;                 // [1] norm recompute flag of the panel
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   bool zero_pivot = false, bad = false;
 
@@ -394,8 +422,8 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
     }
     __syncthreads();
     if (wave == 0) {
-      if (nb == 16) qr_panel_wave<16>(pnl, N, j0, w, perm, swp, taus, red, zero_pivot);
-      else qr_panel_wave<8>(pnl, N, j0, w, perm, swp, taus, red, zero_pivot);
+      if (nb == 16) qr_panel_wave<16>(pnl, N, j0, w, perm, swp, taus, red, pnrm, porg, pflg, zero_pivot);
+      else qr_panel_wave<8>(pnl, N, j0, w, perm, swp, taus, red, pnrm, porg, pflg, zero_pivot);
       if (lane == 0) {
         for (int q = 0; q < nb; ++q) fp[q] = q;
         for (int q = 0; q < w; ++q) {
@@ -537,11 +565,18 @@ __global__ __launch_bounds__(QF_THREADS) void k_trsm_perm(const TrsmTask* __rest
       acc[v] = (row < b && cok) ? Z[(size_t)(i0 + row) * N + c] : 0.0;
     }
     const int ra = i0 + li;
-    for (int k = i0 + b; k < N; k += 4) {
-      const int kr = k + lk;
-      const double a = -ldg0(Rg, (size_t)ra * N + kr, li < b && kr < N);
-      const double y = ldg0(Zg, (size_t)kr * N + c, kr < N && cok);
-      acc = qr_mma(a, y, acc);
+    constexpr int KB = 8;  // k-steps whose loads are issued together (zero beyond N)
+    for (int k = i0 + b; k < N; k += 4 * KB) {
+      double a[KB], y[KB];
+#pragma unroll
+      for (int u = 0; u < KB; ++u) {
+        const int kr = k + 4 * u + lk;
+        a[u] = -ldg0(Rg, (size_t)ra * N + kr, li < b && kr < N);
+        y[u] = ldg0(Zg, (size_t)kr * N + c, kr < N && cok);
+      }
+      qr_sched_fence();
+#pragma unroll
+      for (int u = 0; u < KB; ++u) acc = qr_mma(a[u], y[u], acc);
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) blk[(lk + 4 * v) * QF_COLS + wave * 16 + li] = acc[v];
@@ -573,7 +608,7 @@ __global__ __launch_bounds__(QF_THREADS) void k_trsm_perm(const TrsmTask* __rest
 
 size_t qr_blocked_shmem(int nmax) {
   const size_t nb = nmax > 256 ? 8 : 16;
-  return sizeof(double) * (nmax * (nb + 1) + nb * nb + 256 + 16 + 2 * (size_t)nmax) + sizeof(int) * (nmax + 32);
+  return sizeof(double) * (nmax * (nb + 1) + nb * nb + 256 + 48 + 2 * (size_t)nmax) + sizeof(int) * (nmax + 33);
 }
 
 size_t qr_formq_shmem(int nmax) {

@@ -2,7 +2,7 @@
 // CPU build: graded test matrices W = M diag(d) (d over many decades, like the
 // UDV chain), checks W = Q diag(D) V'' (no pivot request), W P = Q diag(D) R' (pivot
 // order requested), Q^T Q = I and the permuted triangular solve.
-// usage: qr_harness N decades seed
+// usage: qr_harness N decades seed [near]
 #include "hip_cpu_shim.h"
 #include "../../latticeqmc_amd/csrc/qr.hip"
 
@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <random>
+#include <string>
 
 using namespace lqmc;
 
@@ -23,6 +24,14 @@ int main(int argc, char** argv) {
   std::vector<double> M(n2), d(N);
   for (auto& x : M) x = g(rng);
   for (int j = 0; j < N; ++j) d[j] = std::pow(10.0, decades * (std::uniform_real_distribution<double>(-0.5, 0.5)(rng)));
+  // "near": columns 2i+1 = columns 2i + 1e-9 noise at the same scale, so a panel that takes one
+  // of a pair as pivot leaves the other with a tiny remaining norm (the recompute safeguard)
+  const bool near = argc > 4 && std::string(argv[4]) == "near";
+  if (near)
+    for (int c = 1; c < N; c += 2) {
+      d[c] = d[c - 1];
+      for (int r = 0; r < N; ++r) M[(size_t)r * N + c] = M[(size_t)r * N + c - 1] + 1e-9 * M[(size_t)r * N + c];
+    }
   double* W = static_cast<double*>(std::malloc(n2 * 8));
   for (int r = 0; r < N; ++r) for (int c = 0; c < N; ++c) W[(size_t)r * N + c] = M[(size_t)r * N + c] * d[c];
   std::vector<double> W0(W, W + n2);
@@ -98,10 +107,18 @@ int main(int argc, char** argv) {
       res = std::max(res, std::fabs(s - Z0[(size_t)r * N + c]));
       zn = std::max(zn, std::fabs(Z0[(size_t)r * N + c]));
     }
+  double ymax = 0.0;
+  for (size_t e = 0; e < n2; ++e) ymax = std::max(ymax, std::fabs(Y[e]));
+  // normwise backward error of the solve: |R' Y' - Z| / (N |R'| |Y'| + |Z|)
+  const double bwd = res / (N * vmax * ymax + zn);
   std::printf("N=%d decades=%g recon(V'') %.2e recon(R') %.2e orth %.2e max|R'| %.3f lower %.1e |diag|-1 %.1e "
-              "trsm_res %.2e flags %u\n", N, decades, recv, rec, orth, vmax, low, dia, res / zn, st.flags);
-  const bool ok = recv < 1e-12 && rec < 1e-12 && orth < 1e-12 && vmax < 1e3 && low == 0.0 && dia < 1e-12 &&
-                  res / zn < 1e-9 && st.flags == 0;
+              "trsm_res %.2e backward %.2e flags %u\n", N, decades, recv, rec, orth, vmax, low, dia, res / zn, bwd,
+              st.flags);
+  // nearly dependent pairs split over two panels leave R' ill-conditioned (max|R'| ~ 1e7): there
+  // the solve is held to its backward error instead of the residual and the |R'| bound
+  const bool solve_ok = near ? bwd < 1e-15 : (vmax < 1e3 && res / zn < 1e-9);
+  const bool ok = recv < 1e-12 && rec < 1e-12 && orth < 1e-12 && low == 0.0 && dia < 1e-12 && solve_ok &&
+                  st.flags == 0;
   std::printf(ok ? "ok\n" : "FAIL\n");
   return ok ? 0 : 1;
 }

@@ -90,12 +90,13 @@ def qr_harness_tsan(tmp_path_factory):
     return _build_qr(tmp_path_factory, "thread")
 
 
-@pytest.mark.parametrize("args", ["4 2 1", "37 20 3", "100 40 5", "257 40 6"])
+@pytest.mark.parametrize("args", ["4 2 1", "37 20 3", "100 40 5", "257 40 6", "64 20 7 near"])
 def test_blocked_qr_and_trsm(qr_harness, args):
     """QR, the column-block Q formation and the permuted triangular solve on graded
     matrices: W = Q diag(D) V'' and W P = Q diag(D) R' to 1e-12 column-relative, Q
     orthogonal, R' upper triangular with unit-modulus diagonal, R' Y' = Z to 1e-9,
-    all accesses in bounds."""
+    all accesses in bounds.  "near" pairs nearly dependent columns, so the panel's
+    downdated column norms collapse and take the exact-recompute path."""
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
     r = subprocess.run([qr_harness, *args.split()], capture_output=True, text=True, env=env, timeout=900)
     assert r.returncode == 0 and "ok" in r.stdout, (r.stdout + r.stderr)[-3000:]
