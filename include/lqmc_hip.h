@@ -65,6 +65,8 @@ typedef struct {
   int32_t flip_threads;/* threads per flip workgroup (0 = automatic) */
   int32_t flip_groups; /* workgroups per chain in the delayed flip, delay 16 or 32 only
                           (0 = automatic: fill the compute units; 1 = one per chain) */
+  int32_t time_displaced; /* Mode D: 1 accumulates G(tau, 0) at every stabilisation point of
+                             the measured sweeps (lqmc_get_green_tau) */
 } lqmc_options;
 
 typedef struct {
@@ -127,6 +129,17 @@ int lqmc_get_green_dev(lqmc_ctx* ctx, int c, double* dev_out, int which, void* s
  * lqmc/lqmc.py:371-375 (which accumulates G only); the reference derives its local
  * moment from the averaged G instead (lqmc/tools.py:87-109). */
 int lqmc_get_spin_corr(lqmc_ctx* ctx, int c, double* out);
+
+/* Unequal-time Green's functions G(tau, 0) = (B(tau,0)^-1 + B(beta,tau))^-1 of chain c, the
+ * mean over the measured sweeps (time_displaced option, Mode D).  They are taken at the
+ * stabilisation points of the sweep, tau = l dtau for l = L-n_s, L-2n_s, ..., 0 (the
+ * slices below the point form B(tau,0)), from the same UDV factors and the same QR as the
+ * stable equal-time G: G(tau,0) = U_R Db_R^-1 X^-1 Ds_L V_L.  ntau (may be NULL) receives
+ * ceil(L/n_s), l_out[ntau] (may be NULL) the slice counts l, out[ntau*2*N*N] (may be NULL;
+ * float64, [tau][spin][i][j]) the means.  No counterpart in lqmc/lqmc.py, whose only
+ * imaginary-time function is the free-fermion one of lqmc/tools.py:182-200; at U = 0 this
+ * is -compute_pole_gf_tau at those tau. */
+int lqmc_get_green_tau(lqmc_ctx* ctx, int c, double* out, int32_t* l_out, int32_t* ntau);
 
 /* Current field of chain c into out[N*L] in the reference layout. */
 int lqmc_get_field(lqmc_ctx* ctx, int c, int8_t* out);

@@ -18,7 +18,8 @@ FLAG_NAMES = {1: "NaN/Inf ratio", 2: "zero pivot", 4: "replay uniforms exhausted
 
 EXPORTS = (
     "lqmc_create", "lqmc_set_chain", "lqmc_init", "lqmc_set_uniforms", "lqmc_sweep",
-    "lqmc_sync", "lqmc_use_graph", "lqmc_get_green", "lqmc_get_green_dev", "lqmc_get_spin_corr", "lqmc_get_field",
+    "lqmc_sync", "lqmc_use_graph", "lqmc_get_green", "lqmc_get_green_dev", "lqmc_get_spin_corr", "lqmc_get_green_tau",
+    "lqmc_get_field",
     "lqmc_get_stats", "lqmc_reset_stats", "lqmc_get_trace", "lqmc_get_observables",
     "lqmc_profile", "lqmc_profile_result", "lqmc_last_error", "lqmc_destroy",
 )
@@ -28,7 +29,7 @@ class Options(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("precision", ctypes.c_int32), ("delay", ctypes.c_int32),
                 ("n_stab", ctypes.c_int32), ("rng", ctypes.c_int32), ("trace", ctypes.c_int32),
                 ("obs_capacity", ctypes.c_int32), ("flip_threads", ctypes.c_int32),
-                ("flip_groups", ctypes.c_int32)]
+                ("flip_groups", ctypes.c_int32), ("time_displaced", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -68,6 +69,7 @@ def load(path=None):
         "lqmc_get_green": (I, [V, I, f64p, I]),
         "lqmc_get_green_dev": (I, [V, I, V, I, V]),
         "lqmc_get_spin_corr": (I, [V, I, P(ctypes.c_double)]),
+        "lqmc_get_green_tau": (I, [V, I, f64p, i32p, i32p]),
         "lqmc_get_field": (I, [V, I, P(ctypes.c_int8)]),
         "lqmc_get_stats": (I, [V, I, P(Stats)]),
         "lqmc_reset_stats": (I, [V]),

@@ -78,6 +78,7 @@ struct Op {
   int nmax;
   int t;
   double work;  // algorithmic work units (flips for OP_FLIP, flops for OP_GEMM)
+  int meas_only = 0;  // skipped in sweeps that do not measure
 };
 
 struct Chain {
@@ -100,6 +101,8 @@ struct Chain {
   double *Tq[2] = {}, *Tt[2] = {}, *aug[2] = {};
   double *Dx[2] = {}, *Zb[2] = {};
   int* px[2] = {};
+  double* Gtd = nullptr;                // [nb][2][N][N] sums of G(tau, 0) (time_displaced)
+  double *Ztd[2] = {}, *Ytd[2] = {};   // its right-hand side and solution
   double *eye = nullptr, *ones = nullptr;
   void* Tw[2] = {};
   void* dws = nullptr;    // delayed-update operands
@@ -212,8 +215,13 @@ static void layout_chain(const lqmc_ctx* x, Chain& c, char* base, size_t& off) {
       carve(base, off, c.Dx[s], N);
       carve(base, off, c.Zb[s], n2);
       carve(base, off, c.px[s], N);
+      if (x->opt.time_displaced) {
+        carve(base, off, c.Ztd[s], n2);
+        carve(base, off, c.Ytd[s], n2);
+      }
     }
   }
+  if (dq && x->opt.time_displaced) carve(base, off, c.Gtd, (size_t)c.nb * 2 * n2);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -436,8 +444,11 @@ static void build_program(lqmc_ctx* x) {
     // stable G from the left entry (or identity) and the right factors:
     //   X = Db_L^-1 (U_L^T U_R) Db_R^-1 + Ds_L (V_L V_R^T) Ds_R ;  X = Qx Dx Vx   (QR)
     //   Z = Dx^-1 Qx^T Db_L^-1 U_L^T ;  Vx Y = Z ;  G = U_R Db_R^-1 Y
+    // td >= 0 (time_displaced, sweep only): also G(tau, 0) = U_R Db_R^-1 X^-1 Ds_L V_L from the
+    // same QR of X, added to the sums of stabilisation point td in measured sweeps:
+    //   Z' = Dx^-1 Qx^T Ds_L V_L ;  Vx Y' = Z' ;  Gtd[td] += U_R Db_R^-1 Y'
     auto recompute = [&](std::vector<Op>& ops, std::vector<int>& which, std::vector<int>& mleft,
-                         std::vector<int>& rvsel, bool right_identity, int compare) {
+                         std::vector<int>& rvsel, bool right_identity, int compare, int td) {
       for (size_t k = 0; k < which.size(); ++k) {
         Chain& c = x->ch[which[k]];
         const size_t n2 = (size_t)c.N * c.N;
@@ -476,6 +487,34 @@ static void build_program(lqmc_ctx* x) {
         }
       }
       b.gemm(ops, g1, 0);
+      if (td >= 0) {  // before the equal-time solve overwrites Qx
+        const size_t from = ops.size();
+        std::vector<GemmTask> gz, gg;
+        std::vector<TrsmTask> tz;
+        for (size_t k = 0; k < which.size(); ++k) {
+          Chain& c = x->ch[which[k]];
+          const size_t n2 = (size_t)c.N * c.N;
+          for (int s = 0; s < 2; ++s) {
+            const bool lid = mleft[k] < 0;
+            const double* DL = lid ? c.ones : c.LD[s] + (size_t)mleft[k] * c.N;
+            const double* VL = lid ? c.eye : c.LV[s] + mleft[k] * n2;
+            const double* UR = right_identity ? c.eye : c.RU[s];
+            const double* DR = right_identity ? c.ones : c.RD[s];
+            GemmTask z = mk(c.aug[s] + n2, c.N, 1, VL, c.N, 0, c.Ztd[s], c.N, c.N);
+            z.rs = c.Dx[s]; z.rs_kind = SC_INV;
+            z.ks = DL; z.ks_kind = SC_SMALL;
+            gz.push_back(z);
+            tz.push_back(TrsmTask{c.aug[s], c.px[s], c.Ztd[s], c.Ytd[s], c.N, which[k]});
+            GemmTask g = mk(UR, c.N, 0, c.Ytd[s], c.N, 0, c.Gtd + ((size_t)td * 2 + s) * n2, c.N, c.N);
+            g.ks = DR; g.ks_kind = SC_INV_BIG; g.beta = 1.0;
+            gg.push_back(g);
+          }
+        }
+        b.gemm(ops, gz, 0);
+        b.trsm(ops, tz);
+        b.gemm(ops, gg, 0);
+        for (size_t i = from; i < ops.size(); ++i) ops[i].meas_only = 1;
+      }
       b.trsm(ops, ts);
       for (size_t k = 0; k < which.size(); ++k) {
         Chain& c = x->ch[which[k]];
@@ -498,7 +537,7 @@ static void build_program(lqmc_ctx* x) {
     {
       std::vector<int> which, ml, rv;
       for (int ci = 0; ci < nc; ++ci) { which.push_back(ci); ml.push_back(0); rv.push_back(0); }
-      recompute(x->init_ops, which, ml, rv, true, 0);
+      recompute(x->init_ops, which, ml, rv, true, 0, -1);
     }
     // sweep
     left_stack(x->sweep_ops, 1);
@@ -559,7 +598,7 @@ static void build_program(lqmc_ctx* x) {
       }
       b.qr(x->sweep_ops, q);
       b.gemm(x->sweep_ops, g1, 0);
-      recompute(x->sweep_ops, bnd, ml, rvsel, false, 1);
+      recompute(x->sweep_ops, bnd, ml, rvsel, false, 1, x->opt.time_displaced ? t / ns : -1);
       for (int ci : bnd)
         if (t == x->ch[ci].L - 1) rvcur[ci] = -1;
     }
@@ -572,6 +611,7 @@ static void build_program(lqmc_ctx* x) {
 static void run_ops(lqmc_ctx* x, const std::vector<Op>& ops, int measure, hipStream_t s, bool profile) {
   const bool fp32 = x->opt.precision == LQMC_FP32;
   for (const Op& op : ops) {
+    if (op.meas_only && !measure) continue;
     int pk = -1;
     if (op.kind == OP_FLIP || op.kind == OP_DET) pk = PK_FLIP;
     else if (op.kind == OP_GEMM) pk = PK_GEMM;
@@ -632,6 +672,9 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
   if (o.delay != 1 && o.delay != 4 && o.delay != 8 && o.delay != 16 && o.delay != 32)
     return fail(LQMC_EINVAL, "delay must be 1 (rank-1) or one of 4, 8, 16, 32 (delayed blocks)");
   if (o.n_stab < 1) return fail(LQMC_EINVAL, "n_stab must be >= 1");
+  if (o.time_displaced && o.mode != LQMC_MODE_DQMC)
+    return fail(LQMC_EINVAL, "time_displaced Green's functions need Mode D (the UDV stabilisation)");
+  o.time_displaced = o.time_displaced ? 1 : 0;
   if (o.mode == LQMC_MODE_REFERENCE && (o.precision != LQMC_FP64 || o.delay != 1))
     return fail(LQMC_EINVAL, "reference mode runs the FP64 rank-1 sweep only");
   if (o.obs_capacity <= 0) o.obs_capacity = 4096;
@@ -986,6 +1029,28 @@ int lqmc_get_spin_corr(lqmc_ctx* x, int ci, double* out) {
   HIPCHK(hipMemcpy(out, c.Sacc, 8 * n2, hipMemcpyDeviceToHost));
   const double m = st.measured ? (double)st.measured : 1.0;
   for (size_t e = 0; e < n2; ++e) out[e] /= m;
+  return LQMC_OK;
+}
+
+int lqmc_get_green_tau(lqmc_ctx* x, int ci, double* out, int32_t* l_out, int32_t* ntau) {
+  if (!x || ci < 0 || ci >= x->nchains) return fail(LQMC_EINVAL, "lqmc_get_green_tau: bad arguments");
+  if (x->opt.mode != LQMC_MODE_DQMC || !x->opt.time_displaced)
+    return fail(LQMC_ESTATE, "context created without the time_displaced option (Mode D only)");
+  HIPCHK(hipSetDevice(x->device));
+  Chain& c = x->ch[ci];
+  const int ns = x->opt.n_stab;
+  if (ntau) *ntau = c.nb;
+  if (l_out)
+    for (int k = 0; k < c.nb; ++k) l_out[k] = c.L - 1 - std::min((k + 1) * ns - 1, c.L - 1);
+  if (out) {
+    const size_t cnt = (size_t)c.nb * 2 * c.N * c.N;
+    ChainState st;
+    HIPCHK(hipStreamSynchronize(x->stream));
+    HIPCHK(hipMemcpy(&st, x->d_state + ci, sizeof st, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(out, c.Gtd, 8 * cnt, hipMemcpyDeviceToHost));
+    const double m = st.measured ? (double)st.measured : 1.0;
+    for (size_t e = 0; e < cnt; ++e) out[e] /= m;
+  }
   return LQMC_OK;
 }
 

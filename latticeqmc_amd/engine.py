@@ -66,10 +66,13 @@ class SweepEngine:
     rng       "replay" (uniforms supplied per sweep) or "philox" (on device)
     flip_groups  workgroups per chain in the delayed flip (delay 16 or 32; 0 = automatic,
               filling the compute units; results do not depend on it)
+    time_displaced  Mode D: accumulate G(tau, 0) at the stabilisation points of measured
+              sweeps (``green_tau``); adds two GEMMs and a triangular solve per point
     """
 
     def __init__(self, chains, mode="dqmc", precision="fp64", delay=1, n_stab=10, rng="replay",
-                 trace=False, device=0, obs_capacity=4096, flip_threads=0, flip_groups=0, graph=False):
+                 trace=False, device=0, obs_capacity=4096, flip_threads=0, flip_groups=0, graph=False,
+                 time_displaced=False):
         self.lib = nat.load()
         self.chains = list(chains)
         if not self.chains:
@@ -77,7 +80,8 @@ class SweepEngine:
         self.mode, self.precision, self.rng = mode, precision, rng
         self.delay, self.n_stab, self.trace_on = int(delay), int(n_stab), bool(trace)
         opt = nat.Options(_MODES[mode], _PREC[precision], int(delay), int(n_stab), _RNG[rng],
-                          int(bool(trace)), int(obs_capacity), int(flip_threads), int(flip_groups))
+                          int(bool(trace)), int(obs_capacity), int(flip_threads), int(flip_groups),
+                          int(bool(time_displaced)))
         ns = (ctypes.c_int32 * len(self.chains))(*[c.n_sites for c in self.chains])
         ls = (ctypes.c_int32 * len(self.chains))(*[c.time_steps for c in self.chains])
         h = ctypes.c_void_p()
@@ -131,6 +135,21 @@ class SweepEngine:
         out = np.empty((n, n), dtype=np.float64)
         nat.check(self.lib.lqmc_get_spin_corr(self.h, i, self._dp(out)), "lqmc_get_spin_corr")
         return out
+
+    def green_tau(self, i):
+        """Unequal-time G(tau, 0) = (B(tau,0)^-1 + B(beta,tau))^-1 of chain i, mean over the
+        measured sweeps, at the stabilisation points tau = l dtau, l = L-n_s, L-2n_s, ..., 0.
+        Returns (l, G) with l int32 (ntau,) and G (ntau, 2, N, N) [tau][spin][i][j]; at U = 0
+        G equals -tools.compute_pole_gf_tau at those tau (tools.py:182-200)."""
+        c = self.chains[i]
+        nt = ctypes.c_int32()
+        nat.check(self.lib.lqmc_get_green_tau(self.h, i, None, None, ctypes.byref(nt)), "lqmc_get_green_tau")
+        ls = np.empty(nt.value, dtype=np.int32)
+        out = np.empty((nt.value, 2, c.n_sites, c.n_sites))
+        nat.check(self.lib.lqmc_get_green_tau(self.h, i, self._dp(out),
+                                              ls.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), None),
+                  "lqmc_get_green_tau")
+        return ls, out
 
     def field(self, i):
         c = self.chains[i]

@@ -17,7 +17,8 @@ Stabilisation schedule (identical in the HIP engine): with recompute points
 l_r(m) = L-1-m*n_s, the sweep keeps a left stack S_m = B(l_r(m))...B(0) in UDV
 form (built at sweep start) and a right product R = B(L-1)...B(l+1) built from
 the updated slices.  After every n_s slices (and after slice 0) G is replaced
-by the stable (I + S R)^-1 and max|G_wrapped - G_stable| is recorded.
+by the stable (I + S R)^-1 and max|G_wrapped - G_stable| is recorded; the unequal-time
+G(tau, 0) = (S^-1 + R)^-1 at that point (tau = l dtau, l slices in S) is recorded too.
 
 UDV: W = (B...B U) diag(D); W P = Q R (pivoted Householder QR);
 D' = |diag R|, V' = (D'^-1 R) P^T V.  Right products are kept transposed,
@@ -53,6 +54,22 @@ def stable_g(left, right):
     x = (ul.T @ ur) / dlb[:, None] / drb[None, :] + dls[:, None] * (vl @ vr.T) * drs[None, :]
     rhs = ul.T / dlb[:, None]
     y = np.linalg.solve(x, rhs)
+    return (ur / drb[None, :]) @ y
+
+
+def stable_gtd(left, right):
+    """Unequal-time G(tau, 0) = (S^-1 + R)^-1 = G(tau) S for the left product S = U_L D_L V_L
+    (slices below tau) and the transposed right product R = V_R^T D_R U_R^T (slices above):
+    factoring V_L^-1 out on the left and U_R^T on the right leaves
+    D_L^-1 (U_L^T U_R) + (V_L V_R^T) D_R = Ds_L^-1 X Db_R with the X of stable_g, so
+    G(tau, 0) = U_R Db_R^-1 X^-1 Ds_L V_L.  No counterpart in lqmc/lqmc.py (its only
+    imaginary-time function is the free-fermion one of lqmc/tools.py:182-200)."""
+    ul, dl, vl = left
+    ur, dr, vr = right
+    dlb, dls = np.maximum(dl, 1.0), np.minimum(dl, 1.0)
+    drb, drs = np.maximum(dr, 1.0), np.minimum(dr, 1.0)
+    x = (ul.T @ ur) / dlb[:, None] / drb[None, :] + dls[:, None] * (vl @ vr.T) * drs[None, :]
+    y = np.linalg.solve(x, dls[:, None] * vl)
     return (ur / drb[None, :]) @ y
 
 
@@ -204,6 +221,7 @@ class DQMCOracle:
         rec = []
         g_meas = None
         errs = []
+        gtd = []  # (l, [G_up(l dtau, 0), G_dn(l dtau, 0)]) at each stabilisation point
         for t in range(L):
             l = L - 1 - t
             if self.delay > 1:
@@ -230,11 +248,14 @@ class DQMCOracle:
                     gst = stable_g(left, right[s_i])
                     err = max(err, float(np.max(np.abs(self.g[s_i] - gst))))
                     self.g[s_i] = gst
+                    if s_i == 0:
+                        gtd.append((l, []))
+                    gtd[-1][1].append(stable_gtd(left, right[s_i]))
                 errs.append(err)
         self.errors.extend(errs)
         ratios = np.array([r for r, _ in rec])
         accs = np.array([a for _, a in rec], dtype=bool)
-        return dict(accept=accs, ratio=ratios, g_meas=g_meas, errors=errs)
+        return dict(accept=accs, ratio=ratios, g_meas=g_meas, errors=errs, gtd=gtd)
 
     def exact_g(self, l, sigma):
         """Direct (unstabilised) G(l) for small beta checks."""

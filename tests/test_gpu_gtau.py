@@ -1,0 +1,90 @@
+"""Unequal-time G(tau, 0) on the GPU (time_displaced option) against the oracle.
+
+The oracle's sweep records G(tau, 0) = (S^-1 + R)^-1 at every stabilisation point
+(oracle/dqmc.py stable_gtd, pinned on the CPU against explicit slice products and the
+U = 0 free-fermion tools).  Bars: per-point means over the measured sweeps within 1e-8
+of the oracle's (relative to max(1, |G|)), with the flips replayed bit for bit; sweeps that
+do not measure add nothing; at U = 0 the means equal -tools.compute_pole_gf_tau; the
+option is refused outside Mode D and the getter without it.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from gpu_cases import masked_equal, spec_from, square_case
+from latticeqmc_amd import tools
+from latticeqmc_amd.engine import SweepEngine
+from oracle import dqmc, model
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("side,L,beta,n_stab,delay", [
+    (4, 16, 2.0, 5, 1), (6, 20, 2.0, 10, 16), (8, 40, 4.0, 10, 8),
+])
+def test_green_tau_matches_oracle(side, L, beta, n_stab, delay):
+    _say(f"[gtau] side={side} L={L} beta={beta} n_stab={n_stab} delay={delay}")
+    k, f = square_case(side, L, beta, 4.0, 5)
+    n = side * side
+    o = dqmc.DQMCOracle(k, beta, L, 4.0, f, n_stab=n_stab)
+    eng = SweepEngine([spec_from(k, beta, L, 4.0, f)], mode="dqmc", delay=delay, n_stab=n_stab,
+                      rng="replay", trace=True, time_displaced=True)
+    rs = np.random.RandomState(7)
+    acc_sum = None
+    for s, measure in enumerate((False, True, True)):
+        uni = rs.rand(L * n)
+        res = o.sweep(uni)
+        eng.set_uniforms(0, uni)
+        eng.sweep(1, measure=measure)
+        eng.sync()
+        ratio, acc = eng.trace(0)
+        ok, _ = masked_equal(acc, res["accept"], res["ratio"], uni)
+        assert ok, f"sweep {s}: accept bits differ"
+        ls, gm = eng.green_tau(0)
+        assert list(ls) == [l for l, _ in res["gtd"]]
+        if not measure:
+            assert not gm.any()  # nothing accumulated outside measured sweeps
+            continue
+        g = np.array([[gu, gd] for _, (gu, gd) in res["gtd"]])
+        acc_sum = g if acc_sum is None else acc_sum + g
+    ls, gm = eng.green_tau(0)
+    want = acc_sum / 2
+    assert np.abs(gm - want).max() <= 1e-8 * max(1.0, np.abs(want).max())
+    assert eng.stats(0)["flags"] == 0
+    eng.close()
+
+
+def test_green_tau_u0_equals_free_fermion_tools():
+    _say("[gtau u0]")
+    k = model.square_k(4, 4) - 0.3 * np.eye(16)
+    L, beta, ns = 16, 2.0, 6
+    eng = SweepEngine([spec_from(k, beta, L, 0.0, model.random_field(16, L, 0))], mode="dqmc", delay=4,
+                      n_stab=ns, rng="philox", graph=True, time_displaced=True)
+    eng.sweep(1)
+    eng.sweep(2, measure=True)
+    eng.sync()
+    ls, gm = eng.green_tau(0)
+    assert list(ls) == [10, 4, 0]
+    _, gt = tools.compute_pole_gf_tau(k, beta, n_tau=L + 1)  # tau_j = j dtau
+    for j, l in enumerate(ls):
+        for s in range(2):
+            assert np.abs(gm[j, s] + gt[:, :, l]).max() < 1e-10
+    eng.close()
+
+
+def test_green_tau_option_checks():
+    _say("[gtau checks]")
+    from latticeqmc_amd._native import LqmcError
+    k, f = square_case(2, 8, 1.0, 4.0, 0)
+    with pytest.raises(LqmcError, match="Mode D"):
+        SweepEngine([spec_from(k, 1.0, 8, 4.0, f)], mode="reference", rng="replay", time_displaced=True)
+    eng = SweepEngine([spec_from(k, 1.0, 8, 4.0, f)], mode="dqmc", n_stab=4, rng="philox")
+    with pytest.raises(LqmcError, match="time_displaced"):
+        eng.green_tau(0)
+    eng.close()

@@ -152,3 +152,46 @@ def test_spin_correlation_matches_fock_space():
     m = np.diag(g_dn) - np.diag(g_up)
     wrong = np.outer(m, m) + sum((np.eye(4) - g) * g for g in (g_up, g_dn))
     assert np.abs(wrong - s_exact).max() > 1e-6
+
+
+def _prod(o, ls, sigma):
+    """B(ls[0]) B(ls[1]) ... built directly."""
+    p = np.eye(o.n)
+    for l in reversed(ls):
+        p = o.bmul_left(l, sigma, p)
+    return p
+
+
+def test_unequal_time_g_matches_direct_inverse():
+    """stable_gtd from the left stack and a right UDV equals (B(tau,0)^-1 + B(beta,tau))^-1
+    built from explicit slice products, and G(tau) B(tau,0); a transposed factor, a missing
+    V_L or the other half of the D split would each break it."""
+    o = _case(side=2, L=16, beta=2.0, n_stab=4)
+    for sg in (1, -1):
+        st = o.left_stack(sg, 0)
+        for m in range(1, o.nblk):
+            tau = o.block_range(m)[1] + 1  # S_m = B(tau-1) ... B(0)
+            s_mat = _prod(o, list(range(tau - 1, -1, -1)), sg)
+            r_mat = _prod(o, list(range(o.L - 1, tau - 1, -1)), sg)  # B(L-1) ... B(tau)
+            q, d, t = dqmc.udv_qr(r_mat.T)  # R = t^T diag(d) q^T: the transposed right UDV
+            gtd = dqmc.stable_gtd(st[m], (q, d, t))
+            assert np.abs(gtd - np.linalg.inv(np.linalg.inv(s_mat) + r_mat)).max() < 1e-10
+            g_eq = np.linalg.inv(np.eye(o.n) + s_mat @ r_mat)
+            assert np.abs(gtd - g_eq @ s_mat).max() < 1e-10
+            dl = st[m][1]
+            assert dl.max() > 1.0 > dl.min()  # both halves of the D split are exercised
+
+
+def test_unequal_time_g_u0_against_free_fermion_tools():
+    """At U = 0 the sweep's G(tau, 0) is e^{-tau K} (I + e^{-beta K})^-1, minus the
+    free-fermion pole Green's function of the tools module at the same tau."""
+    from latticeqmc_amd import tools
+    k = model.square_k(4, 4) - 0.3 * np.eye(16)
+    L, beta, ns = 16, 2.0, 6
+    o = dqmc.DQMCOracle(k, beta, L, 0.0, model.random_field(16, L, 0), n_stab=ns)
+    res = o.sweep(np.random.RandomState(2).rand(16 * L))
+    _, gt = tools.compute_pole_gf_tau(k, beta, n_tau=L + 1)  # tau_j = j dtau
+    assert [l for l, _ in res["gtd"]] == [10, 4, 0]
+    for l, (gu, gd) in res["gtd"]:
+        assert np.abs(gu + gt[:, :, l]).max() < 1e-10
+        assert np.abs(gd + gt[:, :, l]).max() < 1e-10
