@@ -193,6 +193,7 @@ __device__ inline void qr_panel_gram(const double* pnl, int N, int r0, int q, do
     const double x = SQ ? 1.0 : row[q];
 #pragma unroll
     for (int c0 = 0; c0 < NB; c0 += 8) {  // batches of 8 loads in flight
+      if (!SQ && c0 + 8 <= q) continue;   // columns before the pivot are not needed
       double rv[8];
 #pragma unroll
       for (int c = 0; c < 8; ++c) rv[c] = row[c0 + c];
@@ -281,6 +282,7 @@ __device__ void qr_panel_wave(double* pnl, int N, int j0, int w, int* perm, int*
       const double v = r == j ? 1.0 : row[q] * scal;
 #pragma unroll
       for (int c0 = 0; c0 < NB; c0 += 8) {
+        if (c0 + 8 <= q) continue;  // columns before the pivot do not change
         double rv[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c) rv[c] = row[c0 + c];
@@ -464,7 +466,7 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
       const int p = e / nb, q = e % nb;
       tout[e] = (p < w && q < w) ? Tm[e] : 0.0;
     }
-    if (j0 + w < N) qr_apply_block<QR_WAVES, 1, 1>(A, N, j0, j0 + w, N, pnl, ps, w, Tm, nb, true);
+    if (j0 + w < N) qr_apply_block<QR_WAVES, 8, 1>(A, N, j0, j0 + w, N, pnl, ps, w, Tm, nb, true);
   }
 
   // D = |diag R|; V'' = D^-1 R P^T over W, or R' = D^-1 R (unpermuted) with the pivot order
