@@ -54,8 +54,8 @@ __device__ inline void qr_sched_fence() { __builtin_amdgcn_sched_barrier(0); }
 // update -- to keep many global loads in flight instead of waiting on each.  That pays
 // where a few wavefronts per workgroup share a compute unit with other workgroups (Q
 // formation: KB = 8, RB = 4); with 16 wavefronts of one factorisation all streaming tiles
-// the compute unit's memory path is the limit and the larger batches only add register
-// pressure (KB = RB = 1).  Ends with a workgroup barrier.
+// the compute unit's memory path is the limit, and row batches only add register pressure
+// (KB = 8, RB = 1).  Ends with a workgroup barrier.
 template <int NW, int KB, int RB>
 __device__ void qr_apply_block(double* X, int N, int r0, int c_begin, int c_end, const double* pnl, int ps,
                                int w, const double* Tm, int nb, bool trans) {
