@@ -37,6 +37,14 @@ class BatchedDQMC:
     def observables(self, i):
         return self.engine.observables(i)
 
+    def green_tau(self, i):
+        """Unequal-time G(tau, 0) of chain i (engine option ``time_displaced=True``), mean over
+        the measured sweeps at the stabilisation points: (tau, G) with tau = l * dtau
+        decreasing to 0 and G of shape (ntau, 2, N, N)."""
+        ls, g = self.engine.green_tau(i)
+        c = self.specs[i]
+        return ls * (c.beta / c.time_steps), g
+
     def stats(self, i=-1):
         return self.engine.stats(i)
 

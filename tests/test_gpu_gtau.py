@@ -64,18 +64,18 @@ def test_green_tau_u0_equals_free_fermion_tools():
     _say("[gtau u0]")
     k = model.square_k(4, 4) - 0.3 * np.eye(16)
     L, beta, ns = 16, 2.0, 6
-    eng = SweepEngine([spec_from(k, beta, L, 0.0, model.random_field(16, L, 0))], mode="dqmc", delay=4,
-                      n_stab=ns, rng="philox", graph=True, time_displaced=True)
-    eng.sweep(1)
-    eng.sweep(2, measure=True)
-    eng.sync()
-    ls, gm = eng.green_tau(0)
-    assert list(ls) == [10, 4, 0]
-    _, gt = tools.compute_pole_gf_tau(k, beta, n_tau=L + 1)  # tau_j = j dtau
-    for j, l in enumerate(ls):
+    from latticeqmc_amd.batched import BatchedDQMC
+    bd = BatchedDQMC([spec_from(k, beta, L, 0.0, model.random_field(16, L, 0))], mode="dqmc", delay=4,
+                     n_stab=ns, graph=True, time_displaced=True)
+    bd.run(1, 2)
+    tau, gm = bd.green_tau(0)
+    assert np.allclose(tau, np.array([10, 4, 0]) * beta / L)
+    tgrid, gt = tools.compute_pole_gf_tau(k, beta, n_tau=L + 1)  # tau_j = j dtau
+    for j, l in enumerate([10, 4, 0]):
+        assert np.isclose(tgrid[l], tau[j])
         for s in range(2):
             assert np.abs(gm[j, s] + gt[:, :, l]).max() < 1e-10
-    eng.close()
+    bd.close()
 
 
 def test_green_tau_option_checks():
@@ -87,4 +87,36 @@ def test_green_tau_option_checks():
     eng = SweepEngine([spec_from(k, 1.0, 8, 4.0, f)], mode="dqmc", n_stab=4, rng="philox")
     with pytest.raises(LqmcError, match="time_displaced"):
         eng.green_tau(0)
+    eng.close()
+
+
+def test_green_tau_uneven_chains():
+    """Two chains of different sizes and slice counts in one context: each keeps its own
+    stabilisation points (ceil(L/n_s) of them) and matches its own oracle."""
+    _say("[gtau uneven]")
+    n_stab = 5
+    cases = [(4, 16, 2.0, 11), (3, 12, 1.5, 12)]
+    specs, oracles = [], []
+    for side, L, beta, seed in cases:
+        k, f = square_case(side, L, beta, 4.0, seed)
+        specs.append(spec_from(k, beta, L, 4.0, f))
+        oracles.append(dqmc.DQMCOracle(k, beta, L, 4.0, f, n_stab=n_stab))
+    eng = SweepEngine(specs, mode="dqmc", delay=4, n_stab=n_stab, rng="replay", time_displaced=True)
+    rs = np.random.RandomState(9)
+    sums = [None, None]
+    for _ in range(2):
+        for c, ((side, L, _, _), o) in enumerate(zip(cases, oracles)):
+            uni = rs.rand(L * side * side)
+            res = o.sweep(uni)
+            eng.set_uniforms(c, uni)
+            g = np.array([[gu, gd] for _, (gu, gd) in res["gtd"]])
+            sums[c] = g if sums[c] is None else sums[c] + g
+        eng.sweep(1, measure=True)
+        eng.sync()
+    for c, ((side, L, _, _), o) in enumerate(zip(cases, oracles)):
+        ls, gm = eng.green_tau(c)
+        assert list(ls) == [L - 1 - min((j + 1) * n_stab - 1, L - 1) for j in range(-(-L // n_stab))]
+        assert np.array_equal(eng.field(c), o.field)
+        want = sums[c] / 2
+        assert np.abs(gm - want).max() <= 1e-8 * max(1.0, np.abs(want).max())
     eng.close()
