@@ -385,9 +385,16 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
       const int c = ct * 64 + lane;
       double s = 0.0;
       if (c < N)
-        for (int r = g; r < N; r += ng) {
-          const double x = W[(size_t)r * N + c];
-          s += x * x;
+        for (int r0 = g; r0 < N; r0 += 8 * ng) {  // 8 loads in flight, summed in row order
+          double x[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int r = r0 + u * ng;
+            x[u] = ldg0(gptr(W), (size_t)r * N + c, r < N);
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (r0 + u * ng < N) s += x[u] * x[u];
         }
       if (c < N) pnl[g * N + c] = s;
     }
@@ -410,9 +417,20 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
     perm[rank] = c;
   }
   __syncthreads();
-  for (size_t e = tid; e < (size_t)N * N; e += QR_THREADS) {
-    const int r = (int)(e / N), j = (int)(e % N);
-    A[e] = W[(size_t)r * N + perm[j]];
+  const size_t n2 = (size_t)N * N;
+  for (size_t e0 = tid; e0 < n2; e0 += 8 * QR_THREADS) {  // 8 loads in flight per thread
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const size_t e = e0 + (size_t)u * QR_THREADS;
+      const bool in = e < n2;
+      v[u] = ldg0(gptr(W), in ? (e / N) * N + perm[e % N] : 0, in);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const size_t e = e0 + (size_t)u * QR_THREADS;
+      if (e < n2) A[e] = v[u];
+    }
   }
   __syncthreads();
 
@@ -477,15 +495,21 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
     if (tk.perm) tk.perm[i] = perm[i];
   }
   __syncthreads();
-  if (tk.perm) {
-    for (size_t e = tid; e < (size_t)N * N; e += QR_THREADS) {
-      const int i = (int)(e / N), c = (int)(e % N);
-      W[e] = c >= i ? A[e] / norms[i] : 0.0;
+  for (size_t e0 = tid; e0 < n2; e0 += 8 * QR_THREADS) {  // 8 loads in flight per thread
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const size_t e = e0 + (size_t)u * QR_THREADS;
+      v[u] = ldg0(gptr(A), e, e < n2 && (int)(e % N) >= (int)(e / N));
     }
-  } else {
-    for (size_t e = tid; e < (size_t)N * N; e += QR_THREADS) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const size_t e = e0 + (size_t)u * QR_THREADS;
+      if (e >= n2) continue;
       const int i = (int)(e / N), c = (int)(e % N);
-      W[(size_t)i * N + perm[c]] = c >= i ? A[e] / norms[i] : 0.0;
+      const double y = c >= i ? v[u] / norms[i] : 0.0;
+      if (tk.perm) W[e] = y;
+      else W[(size_t)i * N + perm[c]] = y;
     }
   }
   if (st) {
@@ -517,12 +541,22 @@ __global__ __launch_bounds__(QF_THREADS) void k_qr_formq(const QrTask* __restric
   }
   for (int j0 = ((c1 - 1) / nb) * nb; j0 >= 0; j0 -= nb) {
     const int w = min(nb, N - j0);
-    for (int e = tid; e < (N - j0) * nb; e += QF_THREADS) {
-      const int r = j0 + e / nb, q = e % nb;
-      const int rr = r - j0;
-      double v = 0.0;
-      if (q < w) v = rr < q ? 0.0 : (rr == q ? 1.0 : A[(size_t)r * N + j0 + q]);
-      pnl[r * ps + q] = v;
+    const int tot = (N - j0) * nb;
+    for (int e0 = tid; e0 < tot; e0 += 8 * QF_THREADS) {  // 8 loads in flight per thread
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * QF_THREADS;
+        const int q = e % nb, rr = e / nb;
+        const bool in = e < tot && q < w;
+        const double x = ldg0(gptr(A), (size_t)(j0 + rr) * N + j0 + q, in && rr > q);
+        v[u] = (in && rr == q) ? 1.0 : x;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * QF_THREADS;
+        if (e < tot) pnl[(j0 + e / nb) * ps + e % nb] = v[u];
+      }
     }
     const double* tin = tk.Tt + (size_t)(j0 / nb) * nb * nb;
     for (int e = tid; e < nb * nb; e += QF_THREADS) Tm[e] = tin[e];
@@ -601,9 +635,20 @@ __global__ __launch_bounds__(QF_THREADS) void k_trsm_perm(const TrsmTask* __rest
     __syncthreads();
   }
   const int* __restrict__ perm = tk.perm;
-  for (int e = tid; e < N * QF_COLS; e += QF_THREADS) {
-    const int r = e / QF_COLS, cc = c0 + e % QF_COLS;
-    if (cc < c1) tk.Y[(size_t)perm[r] * N + cc] = Z[(size_t)r * N + cc];
+  for (int e0 = tid; e0 < N * QF_COLS; e0 += 8 * QF_THREADS) {  // 8 loads in flight per thread
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + u * QF_THREADS;
+      const int r = e / QF_COLS, cc = c0 + e % QF_COLS;
+      v[u] = ldg0(Zg, (size_t)r * N + cc, e < N * QF_COLS && cc < c1);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + u * QF_THREADS;
+      const int r = e / QF_COLS, cc = c0 + e % QF_COLS;
+      if (e < N * QF_COLS && cc < c1) tk.Y[(size_t)perm[r] * N + cc] = v[u];
+    }
   }
   if (zero && st) atomicOr(&st[tk.chain].flags, 2u);
 }
