@@ -37,7 +37,7 @@ log = logging.getLogger("latticeqmc_amd")
 class LatticeQMC:
     def __init__(self, model, beta, time_steps, warmup=300, sweeps=2000, det_mode=False, log_lvl=None,
                  *, mode="dqmc", precision="fp64", delay=1, n_stab=10, rng="numpy", seed=None,
-                 device=0, graph=False, console=True):
+                 device=0, graph=False, console=True, time_displaced=False):
         if log_lvl is not None:
             log.setLevel(log_lvl)
         self.model = model
@@ -65,6 +65,7 @@ class LatticeQMC:
         self.device = device
         self.graph = graph
         self.console = console
+        self.time_displaced = bool(time_displaced)
         self._engine = None
         self.set_beta(beta)
 
@@ -103,7 +104,8 @@ class LatticeQMC:
                              seed=self.seed, expk=self.exp_k, expk_inv=self.exp_k_inv, lam=float(self.lamb))
             self._engine = SweepEngine([spec], mode=self.mode, precision=self.precision, delay=self.delay,
                                        n_stab=self.n_stab, rng="philox" if self.rng == "philox" else "replay",
-                                       device=self.device, graph=self.graph)
+                                       device=self.device, graph=self.graph,
+                                       time_displaced=self.time_displaced and self.mode == "dqmc")
         return self._engine
 
     def _sweep(self, measure):
@@ -186,3 +188,11 @@ class LatticeQMC:
 
     def observables(self):
         return self._get_engine().observables(0)
+
+    def green_tau(self):
+        """Unequal-time G(tau, 0) (``time_displaced=True``, Mode D): (tau, G) with tau = l dtau at
+        the stabilisation points and G (ntau, 2, N, N), the mean over every measured sweep of
+        this solver.  No counterpart in lqmc/lqmc.py; at U = 0 it is minus the free-fermion
+        ``tools.compute_pole_gf_tau`` (tools.py:182-200) at those tau."""
+        ls, g = self._get_engine().green_tau(0)
+        return ls * self.dtau, g

@@ -120,3 +120,21 @@ def test_green_tau_uneven_chains():
         want = sums[c] / 2
         assert np.abs(gm - want).max() <= 1e-8 * max(1.0, np.abs(want).max())
     eng.close()
+
+
+def test_drop_in_green_tau_u0():
+    """LatticeQMC(time_displaced=True): after run(), green_tau() at U = 0 is minus the
+    free-fermion pole Green's function of the model's kinetic matrix at tau = l dtau."""
+    _say("[gtau drop-in]")
+    from latticeqmc_amd import HubbardModel, LatticeQMC
+    m = HubbardModel(u=0, t=1, mu=0.4)
+    m.build_square(4)
+    q = LatticeQMC(m, beta=2.0, time_steps=16, warmup=1, sweeps=2, n_stab=6, rng="philox", seed=3,
+                   console=False, time_displaced=True)
+    q.run()
+    tau, g = q.green_tau()
+    tgrid, gt = tools.compute_pole_gf_tau(q.ham_kin, 2.0, n_tau=17)
+    assert np.allclose(tau, tgrid[[10, 4, 0]])
+    for j, l in enumerate([10, 4, 0]):
+        for s in range(2):
+            assert np.abs(g[j, s] + gt[:, :, l]).max() < 1e-10
