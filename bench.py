@@ -85,7 +85,11 @@ def time_form(specs, steps, warmup, profile_kind=None, **kw):
     from latticeqmc_amd.engine import SweepEngine
     eng = SweepEngine(specs, rng="philox", **kw)
     upd = sum(s.n_sites * s.time_steps for s in specs)
-    eng.sweep(warmup, measure=False)
+    # the warmup ends with a measured sweep, so both sweep graphs (with and without the
+    # measurement) are captured and instantiated before the timed region
+    if warmup > 1:
+        eng.sweep(warmup - 1, measure=False)
+    eng.sweep(1, measure=True)
     eng.sync()
     eng.reset_stats()
     sync_all()
@@ -221,7 +225,7 @@ def main():
     fp32 = None
     if not args.no_fp32 and args.precision != "fp32":
         fp32 = time_form(specs, 1, 1, profile_kind=None, mode="dqmc", precision="fp32", delay=args.delay,
-                         n_stab=args.n_stab, graph=not args.no_graph)
+                         n_stab=args.n_stab, graph=not args.no_graph, max_dg=-1.0)
         log(f"[bench] fp32 form: {fp32['ms_per_step']:.1f} ms/sweep, max|dG| {fp32['stats']['max_err']:.2e}")
     st = chosen["stats"]
     out = {

@@ -22,6 +22,7 @@
 #ifndef LQMC_HIP_H
 #define LQMC_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -50,6 +51,7 @@ enum {
   LQMC_FLAG_REPLAY_UNDERRUN = 4,/* a replay sweep ran past the uploaded uniforms */
   LQMC_FLAG_NAN_G = 8,          /* a recompute found NaN/Inf in G */
   LQMC_FLAG_FLIP_BARRIER = 16,  /* a wait between the workgroups of one chain's flip timed out */
+  LQMC_FLAG_PRECISION = 32,     /* a recompute found max|G_propagated - G_recomputed| > max_dg */
 };
 
 typedef struct {
@@ -67,6 +69,10 @@ typedef struct {
                           (0 = automatic: fill the compute units; 1 = one per chain) */
   int32_t time_displaced; /* Mode D: 1 accumulates G(tau, 0) at every stabilisation point of
                              the measured sweeps (lqmc_get_green_tau) */
+  double max_dg;       /* Mode D: a recompute whose max|G_propagated - G_recomputed| exceeds this
+                          raises LQMC_FLAG_PRECISION, so lqmc_sync fails (0 = 1e-3; < 0 = off).
+                          The Metropolis ratios between recomputes read the propagated G, so a
+                          large deviation (an FP32 form at low temperature) biases the chain. */
 } lqmc_options;
 
 typedef struct {
@@ -106,10 +112,13 @@ int lqmc_set_uniforms(lqmc_ctx* ctx, int c, const double* u, int64_t n_sweeps);
 /* Runs n_sweeps sweeps of every chain (_update_step, lqmc/lqmc.py:301-347, and
  * the loops warmup_loop/measure_loop 349-375). measure != 0 accumulates the
  * end-of-sweep G (lqmc/lqmc.py:371-372) and per-sweep observables.  Asynchronous
- * on the context stream; stream may be NULL (context stream) or a hipStream_t. */
+ * on the context stream; stream may be NULL (context stream) or a hipStream_t.  On a
+ * caller stream the context stream is made to wait for the sweep's end, so every later
+ * call on the context (getters, lqmc_set_uniforms, lqmc_sync) sees its results. */
 int lqmc_sweep(lqmc_ctx* ctx, int n_sweeps, int measure, void* stream);
 
-/* Waits for the context stream and checks the device error flags. */
+/* Waits for the context stream (only: other work on the device is not waited for) and
+ * checks the device error flags. */
 int lqmc_sync(lqmc_ctx* ctx);
 
 /* Enables (1) or disables (0) replaying whole sweeps from captured hipGraphs. */
@@ -120,8 +129,18 @@ int lqmc_use_graph(lqmc_ctx* ctx, int on);
  * measure_loop, lqmc/lqmc.py:375); 2: G of the last measured sweep. */
 int lqmc_get_green(lqmc_ctx* ctx, int c, double* out, int which);
 
-/* Same, into device memory (e.g. a torch tensor's data_ptr()), on the given stream. */
+/* Same, into device memory (e.g. a torch tensor's data_ptr()): a device-to-device copy
+ * (converted to float64, divided by the measured-sweep count for which = 1) issued on
+ * stream (NULL: the context stream) after the work queued on the context stream.
+ * Asynchronous: dev_out is complete once that stream is synchronised. */
 int lqmc_get_green_dev(lqmc_ctx* ctx, int c, double* dev_out, int which, void* stream);
+
+/* Mode D: max |G_propagated - G_recomputed| at each recompute point of the sweep since
+ * create / lqmc_reset_stats, out[k] for the k-th point (after slices L-1 .. L-(k+1)n_s, the
+ * last after slice 0); *n_points (may be NULL) receives ceil(L/n_s), out may be NULL.  The
+ * per-point form of lqmc_stats.max_err: how far the propagation drifts from the FP64
+ * stabilised G (lqmc/lqmc.py:303-307 recomputes G once per sweep without this check). */
+int lqmc_get_recompute_err(lqmc_ctx* ctx, int c, double* out, int32_t* n_points);
 
 /* Mean over the measured sweeps of the equal-time spin correlation <S^z_i S^z_j>
  * (S^z = n_up - n_dn, out N x N row-major), estimated from the end-of-sweep G of each
@@ -163,6 +182,20 @@ int lqmc_get_observables(lqmc_ctx* ctx, int c, double* out, int max_rows, int* r
  * 1 = GEMM, 2 = QR, 3 = solve; kind = -1 stops profiling. */
 int lqmc_profile(lqmc_ctx* ctx, int kind);
 int lqmc_profile_result(lqmc_ctx* ctx, double* avg_ms, int64_t* launches, double* work_units);
+
+/* Slice wrap of Green's functions held by the caller, outside a context: for m = 0 ..
+ * count-1, G_m <- B_m^-1 G_m B_m with B_m = diag(v_m) E_K, the wrap of _update_step
+ * (lqmc/lqmc.py:338-345) in the consistent form the Mode D sweep applies after each slice
+ * (the reference's own W = V(l-1) E_K wraps with the wrong slice and direction).  All
+ * pointers are device memory: expk = E_K and expk_inv = E_K^-1 (n x n, row-major), v
+ * (count x n: the diagonal of V for each matrix, e.g. exp(-sigma lambda s[:, l]) of both
+ * spins), g (count x n x n, in/out), and workspace of at least
+ * lqmc_wrap_green_workspace(n, count) bytes (descriptors and one n x n temporary per
+ * matrix).  Two batched FP64 matrix-core products; asynchronous on stream (NULL: the
+ * null stream), nothing is copied from the host. */
+size_t lqmc_wrap_green_workspace(int n, int count);
+int lqmc_wrap_green(int device, int n, int count, const double* expk, const double* expk_inv, const double* v,
+                    double* g, void* workspace, size_t ws_bytes, void* stream);
 
 const char* lqmc_last_error(void);
 int lqmc_destroy(lqmc_ctx* ctx);

@@ -14,14 +14,16 @@ FP64, FP32 = 0, 1
 RNG_REPLAY, RNG_PHILOX = 0, 1
 
 ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "ESTATE", -5: "ENUMERIC"}
-FLAG_NAMES = {1: "NaN/Inf ratio", 2: "zero pivot", 4: "replay uniforms exhausted", 8: "NaN/Inf in G"}
+FLAG_NAMES = {1: "NaN/Inf ratio", 2: "zero pivot", 4: "replay uniforms exhausted", 8: "NaN/Inf in G",
+              16: "flip barrier timeout", 32: "recompute deviation above max_dg"}
 
 EXPORTS = (
     "lqmc_create", "lqmc_set_chain", "lqmc_init", "lqmc_set_uniforms", "lqmc_sweep",
-    "lqmc_sync", "lqmc_use_graph", "lqmc_get_green", "lqmc_get_green_dev", "lqmc_get_spin_corr", "lqmc_get_green_tau",
+    "lqmc_sync", "lqmc_use_graph", "lqmc_get_green", "lqmc_get_green_dev", "lqmc_get_recompute_err", "lqmc_get_spin_corr", "lqmc_get_green_tau",
     "lqmc_get_field",
     "lqmc_get_stats", "lqmc_reset_stats", "lqmc_get_trace", "lqmc_get_observables",
-    "lqmc_profile", "lqmc_profile_result", "lqmc_last_error", "lqmc_destroy",
+    "lqmc_profile", "lqmc_profile_result", "lqmc_wrap_green_workspace", "lqmc_wrap_green",
+    "lqmc_last_error", "lqmc_destroy",
 )
 
 
@@ -29,7 +31,8 @@ class Options(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("precision", ctypes.c_int32), ("delay", ctypes.c_int32),
                 ("n_stab", ctypes.c_int32), ("rng", ctypes.c_int32), ("trace", ctypes.c_int32),
                 ("obs_capacity", ctypes.c_int32), ("flip_threads", ctypes.c_int32),
-                ("flip_groups", ctypes.c_int32), ("time_displaced", ctypes.c_int32)]
+                ("flip_groups", ctypes.c_int32), ("time_displaced", ctypes.c_int32),
+                ("max_dg", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
@@ -68,6 +71,7 @@ def load(path=None):
         "lqmc_use_graph": (I, [V, I]),
         "lqmc_get_green": (I, [V, I, f64p, I]),
         "lqmc_get_green_dev": (I, [V, I, V, I, V]),
+        "lqmc_get_recompute_err": (I, [V, I, f64p, i32p]),
         "lqmc_get_spin_corr": (I, [V, I, P(ctypes.c_double)]),
         "lqmc_get_green_tau": (I, [V, I, f64p, i32p, i32p]),
         "lqmc_get_field": (I, [V, I, P(ctypes.c_int8)]),
@@ -77,6 +81,8 @@ def load(path=None):
         "lqmc_get_observables": (I, [V, I, f64p, I, P(I)]),
         "lqmc_profile": (I, [V, I]),
         "lqmc_profile_result": (I, [V, f64p, P(ctypes.c_int64), f64p]),
+        "lqmc_wrap_green_workspace": (ctypes.c_size_t, [I, I]),
+        "lqmc_wrap_green": (I, [I, I, I, V, V, V, V, V, ctypes.c_size_t, V]),
         "lqmc_last_error": (ctypes.c_char_p, []),
         "lqmc_destroy": (I, [V]),
     }

@@ -186,11 +186,13 @@ def measure_scan(model, u_values, betas, time_steps, warmup=500, sweeps=5000, fo
     if not todo:
         return paths
     specs = []
-    for j, u in enumerate(todo):
+    u_index = {u: j for j, u in enumerate(u_values)}
+    for u in todo:
         model.set_params(u=u)
         kmat = model.ham_kinetic()
         for i, b in enumerate(betas):
-            s = seed + j * len(betas) + i
+            # one seed per (U, beta) pair of the whole scan, however it is split or resumed
+            s = seed + u_index[u] * len(betas) + i
             rs = np.random.RandomState(s)
             field = (2 * rs.randint(0, 2, size=(model.n_sites, time_steps)) - 1).astype(np.int8)
             specs.append(ChainSpec(kmat, float(b), time_steps, model.u, field, seed=s))

@@ -128,10 +128,13 @@ struct SolveTask {
 };
 
 // Compare the propagated G (form precision) with the recomputed Gst (f64),
-// record the max deviation, then overwrite G by Gst.
+// record the max deviation (per chain, and per recompute point in slot), raise the
+// precision flag above tol, then overwrite G by Gst.
 struct FinishTask {
   void* G;
   const double* Gst;
+  unsigned long long* slot;  // max deviation at this recompute point (ordered bits), or null
+  double tol;                // > 0: flag 32 when the deviation exceeds it
   int N;
   int chain;
   int compare;

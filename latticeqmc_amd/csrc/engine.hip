@@ -36,6 +36,8 @@ void launch_solve(const SolveTask* d, int n, int nmax, ChainState* st, hipStream
 void launch_finish(int fp32, const FinishTask* d, int n, int nmax, ChainState* st, hipStream_t s);
 void launch_identity(double* const* ptrs, const int* ns, int n, hipStream_t s);
 void launch_expv(const ChainDev* chains, int n, hipStream_t s);
+void launch_green_out(double* out, const void* src, int fp32, size_t n, const ChainState* st, int mean,
+                      hipStream_t s);
 void launch_flip(int fp32, int delayed, const FlipArgs& a, int nchains, int threads, size_t shmem,
                  hipStream_t s);
 size_t flip_delayed_shmem(int nmax, int k, int threads, int fp32);
@@ -51,6 +53,11 @@ void launch_det_sweep(const ChainDev* chains, ChainState* st, int nchains, int m
 using namespace lqmc;
 
 static thread_local std::string g_err;
+
+namespace lqmc {
+// The thread-local error message of the C ABI, for entry points in other source files.
+void lqmc_set_error(const std::string& msg) { g_err = msg; }
+}  // namespace lqmc
 
 static int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -101,6 +108,7 @@ struct Chain {
   double *Tq[2] = {}, *Tt[2] = {}, *aug[2] = {};
   double *Dx[2] = {}, *Zb[2] = {};
   int* px[2] = {};
+  unsigned long long* rec = nullptr;    // [nb] max |G_wrapped - G_recomputed| per recompute point
   double* Gtd = nullptr;                // [nb][2][N][N] sums of G(tau, 0) (time_displaced)
   double *Ztd[2] = {}, *Ytd[2] = {};   // its right-hand side and solution
   double *eye = nullptr, *ones = nullptr;
@@ -123,6 +131,9 @@ struct lqmc_ctx {
   int nchains = 0;
   lqmc_options opt{};
   hipStream_t stream = nullptr;
+  hipEvent_t done_ev = nullptr;  // end of the last sweep issued on a caller stream
+  hipEvent_t aux_ev = nullptr;   // context-stream point a caller stream waits for
+  double max_dg = 1e-3;          // precision-flag threshold of the recompute check (0: off)
   std::vector<Chain> ch;
   char* arena = nullptr;
   size_t arena_bytes = 0;
@@ -222,6 +233,7 @@ static void layout_chain(const lqmc_ctx* x, Chain& c, char* base, size_t& off) {
     }
   }
   if (dq && x->opt.time_displaced) carve(base, off, c.Gtd, (size_t)c.nb * 2 * n2);
+  if (dq) carve(base, off, c.rec, (size_t)c.nb);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -374,7 +386,7 @@ static void build_program(lqmc_ctx* x) {
       for (int s = 0; s < 2; ++s) {
         SolveTask t{c.aug[s], nullptr, nullptr, c.Gst + (size_t)s * c.N * c.N, c.N, 1, ci, 0};
         sv.push_back(t);
-        fin.push_back(FinishTask{Gp(c, s), c.Gst + (size_t)s * c.N * c.N, c.N, ci, 0, 0});
+        fin.push_back(FinishTask{Gp(c, s), c.Gst + (size_t)s * c.N * c.N, nullptr, 0.0, c.N, ci, 0, 0});
       }
     }
     b.solve(x->sweep_ops, sv);
@@ -447,8 +459,9 @@ static void build_program(lqmc_ctx* x) {
     // td >= 0 (time_displaced, sweep only): also G(tau, 0) = U_R Db_R^-1 X^-1 Ds_L V_L from the
     // same QR of X, added to the sums of stabilisation point td in measured sweeps:
     //   Z' = Dx^-1 Qx^T Ds_L V_L ;  Vx Y' = Z' ;  Gtd[td] += U_R Db_R^-1 Y'
+    // slot >= 0: the recompute point's index along the sweep (per-point deviation record)
     auto recompute = [&](std::vector<Op>& ops, std::vector<int>& which, std::vector<int>& mleft,
-                         std::vector<int>& rvsel, bool right_identity, int compare, int td) {
+                         std::vector<int>& rvsel, bool right_identity, int compare, int td, int slot) {
       for (size_t k = 0; k < which.size(); ++k) {
         Chain& c = x->ch[which[k]];
         const size_t n2 = (size_t)c.N * c.N;
@@ -525,7 +538,8 @@ static void build_program(lqmc_ctx* x) {
           GemmTask g = mk(UR, c.N, 0, c.aug[s] + n2, c.N, 0, c.Gst + s * n2, c.N, c.N);
           g.ks = DR; g.ks_kind = SC_INV_BIG;
           g1.push_back(g);
-          fin.push_back(FinishTask{Gp(c, s), c.Gst + s * n2, c.N, which[k], compare, 0});
+          fin.push_back(FinishTask{Gp(c, s), c.Gst + s * n2, slot >= 0 ? c.rec + slot : nullptr, x->max_dg,
+                                   c.N, which[k], compare, 0});
         }
       }
       b.gemm(ops, g1, 0);
@@ -537,7 +551,7 @@ static void build_program(lqmc_ctx* x) {
     {
       std::vector<int> which, ml, rv;
       for (int ci = 0; ci < nc; ++ci) { which.push_back(ci); ml.push_back(0); rv.push_back(0); }
-      recompute(x->init_ops, which, ml, rv, true, 0, -1);
+      recompute(x->init_ops, which, ml, rv, true, 0, -1, -1);
     }
     // sweep
     left_stack(x->sweep_ops, 1);
@@ -598,7 +612,7 @@ static void build_program(lqmc_ctx* x) {
       }
       b.qr(x->sweep_ops, q);
       b.gemm(x->sweep_ops, g1, 0);
-      recompute(x->sweep_ops, bnd, ml, rvsel, false, 1, x->opt.time_displaced ? t / ns : -1);
+      recompute(x->sweep_ops, bnd, ml, rvsel, false, 1, x->opt.time_displaced ? t / ns : -1, t / ns);
       for (int ci : bnd)
         if (t == x->ch[ci].L - 1) rvcur[ci] = -1;
     }
@@ -678,6 +692,7 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
   if (o.mode == LQMC_MODE_REFERENCE && (o.precision != LQMC_FP64 || o.delay != 1))
     return fail(LQMC_EINVAL, "reference mode runs the FP64 rank-1 sweep only");
   if (o.obs_capacity <= 0) o.obs_capacity = 4096;
+  if (!(o.max_dg == o.max_dg)) return fail(LQMC_EINVAL, "max_dg must be a number");
   int ndev = 0;
   HIPCHK(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return fail(LQMC_EINVAL, "device index out of range");
@@ -686,6 +701,7 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
   x->device = device;
   x->nchains = n_chains;
   x->opt = o;
+  x->max_dg = o.max_dg > 0.0 ? o.max_dg : (o.max_dg == 0.0 ? 1e-3 : 0.0);
   x->ch.resize(n_chains);
   for (int c = 0; c < n_chains; ++c) {
     if (n_sites[c] < 1 || n_sites[c] > 1024 || time_steps[c] < 1) {
@@ -779,6 +795,8 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
   for (auto& c : x->ch) layout_chain(x, c, nullptr, off);
   x->arena_bytes = align_up(off, 4096);
   hipError_t e = hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&x->done_ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&x->aux_ev, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(&x->arena, x->arena_bytes);
   if (e == hipSuccess) e = hipMemset(x->arena, 0, x->arena_bytes);
   if (e == hipSuccess) e = hipMalloc(&x->d_chains, sizeof(ChainDev) * n_chains);
@@ -958,6 +976,11 @@ int lqmc_sweep(lqmc_ctx* x, int n_sweeps, int measure, void* stream) {
     for (int k = 0; k < n_sweeps; ++k) run_ops(x, x->sweep_ops, mi, s, profile);
   }
   HIPCHK(hipGetLastError());
+  if (s != x->stream) {
+    // later work on the context stream (the getters, set_uniforms, sync) follows this sweep
+    HIPCHK(hipEventRecord(x->done_ev, s));
+    HIPCHK(hipStreamWaitEvent(x->stream, x->done_ev, 0));
+  }
   return LQMC_OK;
 }
 
@@ -965,7 +988,6 @@ int lqmc_sync(lqmc_ctx* x) {
   if (!x) return fail(LQMC_EINVAL, "null context");
   HIPCHK(hipSetDevice(x->device));
   HIPCHK(hipStreamSynchronize(x->stream));
-  HIPCHK(hipDeviceSynchronize());
   return check_flags(x);
 }
 
@@ -1008,13 +1030,34 @@ int lqmc_get_green(lqmc_ctx* x, int ci, double* out, int which) {
 
 int lqmc_get_green_dev(lqmc_ctx* x, int ci, double* dev_out, int which, void* stream) {
   if (!x || ci < 0 || ci >= x->nchains || !dev_out) return fail(LQMC_EINVAL, "lqmc_get_green_dev: bad arguments");
+  if (which < 0 || which > 2) return fail(LQMC_EINVAL, "which must be 0, 1 or 2");
   HIPCHK(hipSetDevice(x->device));
-  std::vector<double> v;
-  int rc = green_src(x, ci, which, v);
-  if (rc) return rc;
+  Chain& c = x->ch[ci];
+  const size_t n = 2 * (size_t)c.N * c.N;
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : x->stream;
-  HIPCHK(hipMemcpyAsync(dev_out, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, s));
-  HIPCHK(hipStreamSynchronize(s));
+  if (s != x->stream) {  // after the work already queued on the context stream
+    HIPCHK(hipEventRecord(x->aux_ev, x->stream));
+    HIPCHK(hipStreamWaitEvent(s, x->aux_ev, 0));
+  }
+  const bool fp32 = which == 0 && x->opt.precision == LQMC_FP32;
+  const void* src = which == 0 ? c.G : (which == 1 ? (const void*)c.Gacc : (const void*)c.Glast);
+  launch_green_out(dev_out, src, fp32 ? 1 : 0, n, x->d_state + ci, which == 1 ? 1 : 0, s);
+  HIPCHK(hipGetLastError());
+  return LQMC_OK;
+}
+
+int lqmc_get_recompute_err(lqmc_ctx* x, int ci, double* out, int32_t* n_points) {
+  if (!x || ci < 0 || ci >= x->nchains) return fail(LQMC_EINVAL, "lqmc_get_recompute_err: bad arguments");
+  if (x->opt.mode != LQMC_MODE_DQMC) return fail(LQMC_ESTATE, "recompute deviations exist in Mode D only");
+  HIPCHK(hipSetDevice(x->device));
+  Chain& c = x->ch[ci];
+  if (n_points) *n_points = c.nb;
+  if (out) {
+    std::vector<unsigned long long> b(c.nb);
+    HIPCHK(hipStreamSynchronize(x->stream));
+    HIPCHK(hipMemcpy(b.data(), c.rec, sizeof(unsigned long long) * c.nb, hipMemcpyDeviceToHost));
+    for (int k = 0; k < c.nb; ++k) std::memcpy(out + k, &b[k], sizeof(double));
+  }
   return LQMC_OK;
 }
 
@@ -1102,6 +1145,8 @@ int lqmc_reset_stats(lqmc_ctx* x) {
     s.flags = 0;
   }
   HIPCHK(hipMemcpy(x->d_state, st.data(), sizeof(ChainState) * x->nchains, hipMemcpyHostToDevice));
+  for (auto& c : x->ch)
+    if (c.rec) HIPCHK(hipMemset(c.rec, 0, sizeof(unsigned long long) * c.nb));
   return LQMC_OK;
 }
 
@@ -1179,6 +1224,8 @@ int lqmc_destroy(lqmc_ctx* x) {
   if (x->d_eye_ptrs) (void)hipFree(x->d_eye_ptrs);
   if (x->d_eye_ns) (void)hipFree(x->d_eye_ns);
   if (x->d_bar) (void)hipFree(x->d_bar);
+  if (x->done_ev) (void)hipEventDestroy(x->done_ev);
+  if (x->aux_ev) (void)hipEventDestroy(x->aux_ev);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x;
   return LQMC_OK;

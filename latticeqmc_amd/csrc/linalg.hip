@@ -5,6 +5,8 @@
 // 64 lanes of a wavefront to 64 consecutive columns (coalesced row-major
 // access) and the wavefronts of the workgroup to interleaved row groups; the
 // per-column dot products are combined through LDS.
+#include <algorithm>
+
 #include "common.h"
 
 namespace lqmc {
@@ -147,7 +149,10 @@ __global__ __launch_bounds__(256) void k_finish(const FinishTask* __restrict__ t
   }
   if (bad) set_flag(st, tk.chain, 8u);
   if (threadIdx.x == 0 && tk.compare) {
-    atomicMax(&st[tk.chain].max_err_bits, (unsigned long long)__double_as_longlong(rv[0]));
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(rv[0]);
+    atomicMax(&st[tk.chain].max_err_bits, bits);
+    if (tk.slot) atomicMax(tk.slot, bits);
+    if (tk.tol > 0.0 && !(rv[0] <= tk.tol)) set_flag(st, tk.chain, 32u);
     if (blockIdx.y == 0) atomicAdd(&st[tk.chain].recomputes, 1ull);
   }
 }
@@ -172,6 +177,22 @@ __global__ void k_expv(const ChainDev* __restrict__ chains) {
     ch.evi[e] = 1.0 / up;
     ch.evi[NL + e] = 1.0 / dn;
   }
+}
+
+// One chain's G in FP64 into caller device memory: src in the form's precision, divided by
+// the chain's measured-sweep count when mean != 0 (the accumulated sum -> the mean).
+__global__ void k_green_out(double* out, const void* src, int fp32, size_t n, const ChainState* st, int mean) {
+  const double m = mean ? (double)(st->measured ? st->measured : 1ull) : 1.0;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const double v = fp32 ? (double)static_cast<const float*>(src)[e] : static_cast<const double*>(src)[e];
+    out[e] = mean ? v / m : v;
+  }
+}
+
+void launch_green_out(double* out, const void* src, int fp32, size_t n, const ChainState* st, int mean,
+                      hipStream_t s) {
+  const int blocks = (int)std::min<size_t>((n + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_green_out, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, s, out, src, fp32, n, st, mean);
 }
 
 void launch_solve(const SolveTask* d, int n, int nmax, ChainState* st, hipStream_t s) {

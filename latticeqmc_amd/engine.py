@@ -68,11 +68,13 @@ class SweepEngine:
               filling the compute units; results do not depend on it)
     time_displaced  Mode D: accumulate G(tau, 0) at the stabilisation points of measured
               sweeps (``green_tau``); adds two GEMMs and a triangular solve per point
+    max_dg    Mode D: a recompute deviating from the propagated G by more than this raises
+              a device flag, so ``sync`` fails (0: 1e-3; negative: off)
     """
 
     def __init__(self, chains, mode="dqmc", precision="fp64", delay=1, n_stab=10, rng="replay",
                  trace=False, device=0, obs_capacity=4096, flip_threads=0, flip_groups=0, graph=False,
-                 time_displaced=False):
+                 time_displaced=False, max_dg=0.0):
         self.lib = nat.load()
         self.chains = list(chains)
         if not self.chains:
@@ -81,7 +83,7 @@ class SweepEngine:
         self.delay, self.n_stab, self.trace_on = int(delay), int(n_stab), bool(trace)
         opt = nat.Options(_MODES[mode], _PREC[precision], int(delay), int(n_stab), _RNG[rng],
                           int(bool(trace)), int(obs_capacity), int(flip_threads), int(flip_groups),
-                          int(bool(time_displaced)))
+                          int(bool(time_displaced)), float(max_dg))
         ns = (ctypes.c_int32 * len(self.chains))(*[c.n_sites for c in self.chains])
         ls = (ctypes.c_int32 * len(self.chains))(*[c.time_steps for c in self.chains])
         h = ctypes.c_void_p()
@@ -151,6 +153,21 @@ class SweepEngine:
                   "lqmc_get_green_tau")
         return ls, out
 
+    def recompute_errors(self, i):
+        """Mode D: max |G_propagated - G_recomputed| at each recompute point of the sweep
+        (ceil(L/n_s) values, in sweep order) since creation or ``reset_stats``."""
+        n = ctypes.c_int32()
+        nat.check(self.lib.lqmc_get_recompute_err(self.h, i, None, ctypes.byref(n)), "lqmc_get_recompute_err")
+        out = np.empty(n.value)
+        nat.check(self.lib.lqmc_get_recompute_err(self.h, i, self._dp(out), None), "lqmc_get_recompute_err")
+        return out
+
+    def green_dev(self, i, out_ptr, which=0, stream=None):
+        """Copy G of chain i (float64, (2, N, N)) into device memory at out_ptr (e.g. a torch
+        tensor's data_ptr()) on ``stream`` (None: the engine stream); asynchronous."""
+        nat.check(self.lib.lqmc_get_green_dev(self.h, i, ctypes.c_void_p(out_ptr), int(which), stream),
+                  "lqmc_get_green_dev")
+
     def field(self, i):
         c = self.chains[i]
         out = np.empty((c.n_sites, c.time_steps), dtype=np.int8)
@@ -206,3 +223,26 @@ class SweepEngine:
             self.close()
         except Exception:
             pass
+
+
+def wrap_green(expk, expk_inv, v, g, workspace=None, stream=None, device=0):
+    """Slice wrap G_m <- B_m^-1 G_m B_m, B_m = diag(v_m) E_K, of caller-held Green's functions
+    on the device (``lqmc_wrap_green``; the consistent form of lqmc/lqmc.py:338-345).
+
+    expk, expk_inv: (N, N) float64 device tensors; v: (count, N); g: (count, N, N), updated
+    in place.  ``workspace`` (a uint8 device tensor of at least
+    ``lqmc_wrap_green_workspace(N, count)`` bytes) is allocated with torch when omitted.
+    Asynchronous on ``stream`` (a raw hipStream_t; None: the null stream)."""
+    import torch
+    lib = nat.load()
+    count, n = int(g.shape[0]), int(g.shape[1])
+    for t, shape in ((expk, (n, n)), (expk_inv, (n, n)), (v, (count, n)), (g, (count, n, n))):
+        if tuple(t.shape) != shape or t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+            raise ValueError(f"expected a contiguous float64 device tensor of shape {shape}")
+    need = lib.lqmc_wrap_green_workspace(n, count)
+    if workspace is None:
+        workspace = torch.empty(need, dtype=torch.uint8, device=g.device)
+    nat.check(lib.lqmc_wrap_green(int(device), n, count, expk.data_ptr(), expk_inv.data_ptr(), v.data_ptr(),
+                                  g.data_ptr(), workspace.data_ptr(), int(workspace.numel()), stream),
+              "lqmc_wrap_green")
+    return g

@@ -44,6 +44,7 @@ def harness_tsan(tmp_path_factory):
     "256 80 32 512 0 1 0 0 60",     # the throughput configuration
     "100 6 16 256 0 2 1 0 200 2",   # two workgroups per chain, ragged, all blocks concurrent
     "100 6 32 256 0 1 5 1 200 3",   # three workgroups per chain, k = 32, measurement slice
+    "256 3 16 512 0 1 2 1 120 4",   # the throughput form: N = 256, 512 threads, four workgroups
 ])
 def test_flip_kernel_in_bounds(harness, args):
     """In bounds under ASan; delayed runs (9 args) must match the rank-1 kernel on
@@ -60,6 +61,7 @@ def test_flip_kernel_in_bounds(harness, args):
     "36 20 32 256 0 1 3 0",
     "64 16 1 256 0 1 0 1",      # rank-1
     "64 3 16 256 0 1 2 1 100 2",  # two workgroups per chain: the chain barrier orders G
+    "256 2 16 512 0 1 1 1 100 2",  # the throughput form's N = 256 on 512 threads, two workgroups
 ])
 def test_flip_kernel_race_free(harness_tsan, args):
     env = dict(os.environ, TSAN_OPTIONS="halt_on_error=0 report_signal_unsafe=0")

@@ -55,8 +55,12 @@ def test_measure_scan_writes_and_skips(tmp_path):
     betas = [1.0, 2.0]
     seen = []
 
+    seeds = {}
+
     def stub(specs, warmup, sweeps):
         seen.append([(s.u, s.beta) for s in specs])
+        for s in specs:
+            seeds[(s.u, s.beta)] = s.seed
         return np.array([np.full((2, 4, 4), 10 * s.u + s.beta) for s in specs])
 
     folder = str(tmp_path / "scan")
@@ -69,6 +73,9 @@ def test_measure_scan_writes_and_skips(tmp_path):
     seen.clear()
     measure_scan(m, [1.0, 4.0, 6.0], betas, 8, 2, 3, folder=folder, runner=stub)
     assert seen == [[(6.0, 1.0), (6.0, 2.0)]]          # existing U files are skipped
+    # each (U, beta) pair keeps the seed of its place in the whole scan, so a resumed scan
+    # runs the same chains as one uninterrupted scan and no two pairs share a stream
+    assert seeds == {(1.0, 1.0): 0, (1.0, 2.0): 1, (4.0, 1.0): 2, (4.0, 2.0): 3, (6.0, 1.0): 4, (6.0, 2.0): 5}
     assert m.u == 6.0 and m.mu == 3.0 and m.t == 1.0   # set_params(u=U) as comp.py
 
 
