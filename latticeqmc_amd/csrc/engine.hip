@@ -30,7 +30,7 @@ namespace lqmc {
 void launch_gemm(int fp32, int cls, const GemmTask* d_tasks, int ntasks, int ntiles, hipStream_t s);
 int gemm_class(const GemmTask& t);
 int gemm_tiles(int M, int N, int* tiles_n);
-void launch_qr_blocked(const QrTask* d, int n, int nmax, ChainState* st, hipStream_t s);
+void launch_qr_blocked(const QrTask* d, int n, int nmax, ChainState* st, unsigned* pub, int max_wg, hipStream_t s);
 void launch_trsm(const TrsmTask* d, int n, int nmax, ChainState* st, hipStream_t s);
 void launch_solve(const SolveTask* d, int n, int nmax, ChainState* st, hipStream_t s);
 void launch_finish(int fp32, const FinishTask* d, int n, int nmax, ChainState* st, hipStream_t s);
@@ -152,6 +152,8 @@ struct lqmc_ctx {
   int flip_threads = 256;
   int flip_groups = 1;           // workgroups per chain of the delayed flip
   unsigned* d_bar = nullptr;     // [nchains][2] their barrier counters
+  unsigned* d_qrpub = nullptr;   // [4 nchains] hand-over counters of the on-chip QR (two per matrix)
+  int cus = 0;                   // compute units of the device
   size_t flip_shmem = 0;
   size_t det_shmem = 0;
   // graphs
@@ -641,7 +643,9 @@ static void run_ops(lqmc_ctx* x, const std::vector<Op>& ops, int measure, hipStr
     const char* d = x->d_desc + op.off;
     switch (op.kind) {
       case OP_GEMM: launch_gemm(op.fp32, op.nmax, reinterpret_cast<const GemmTask*>(d), op.count, op.ntiles, s); break;
-      case OP_QR: launch_qr_blocked(reinterpret_cast<const QrTask*>(d), op.count, op.nmax, x->d_state, s); break;
+      case OP_QR:
+        launch_qr_blocked(reinterpret_cast<const QrTask*>(d), op.count, op.nmax, x->d_state, x->d_qrpub, x->cus, s);
+        break;
       case OP_TRSM: launch_trsm(reinterpret_cast<const TrsmTask*>(d), op.count, op.nmax, x->d_state, s); break;
       case OP_SOLVE: launch_solve(reinterpret_cast<const SolveTask*>(d), op.count, op.nmax, x->d_state, s); break;
       case OP_FINISH: launch_finish(op.fp32, reinterpret_cast<const FinishTask*>(d), op.count, op.nmax, x->d_state, s); break;
@@ -751,6 +755,7 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
       return fail(LQMC_EHIP, "hipGetDeviceProperties failed");
     }
     const int cus = prop.multiProcessorCount;
+    x->cus = cus;
     const int padded = 8 * ((n_chains + 7) / 8);
     const bool splittable = o.mode == LQMC_MODE_DQMC && (o.delay == 16 || o.delay == 32);
     const int waves = x->flip_threads / 64;
@@ -784,7 +789,7 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
     }
   }
   // The blocked QR keeps one column panel of the largest chain in LDS.
-  if (o.mode == LQMC_MODE_DQMC && qr_blocked_shmem(x->nmax) > (size_t)maxsh) {
+  if (o.mode == LQMC_MODE_DQMC && x->nmax > 256 && qr_blocked_shmem(x->nmax) > (size_t)maxsh) {
     char buf[160];
     snprintf(buf, sizeof buf, "QR kernel needs %zu B of LDS (> %d) for N = %d; reduce the lattice size",
              qr_blocked_shmem(x->nmax), maxsh, x->nmax);
@@ -806,6 +811,8 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
   if (e == hipSuccess) e = hipMalloc(&x->d_eye_ns, sizeof(int) * n_chains);
   if (e == hipSuccess) e = hipMalloc(&x->d_bar, sizeof(unsigned) * 2 * n_chains);
   if (e == hipSuccess) e = hipMemset(x->d_bar, 0, sizeof(unsigned) * 2 * n_chains);
+  if (e == hipSuccess) e = hipMalloc(&x->d_qrpub, sizeof(unsigned) * 4 * n_chains);
+  if (e == hipSuccess) e = hipMemset(x->d_qrpub, 0, sizeof(unsigned) * 4 * n_chains);
   if (e != hipSuccess) {
     std::string m = std::string("allocation failed: ") + hipGetErrorString(e);
     lqmc_destroy(x);
@@ -1224,6 +1231,7 @@ int lqmc_destroy(lqmc_ctx* x) {
   if (x->d_eye_ptrs) (void)hipFree(x->d_eye_ptrs);
   if (x->d_eye_ns) (void)hipFree(x->d_eye_ns);
   if (x->d_bar) (void)hipFree(x->d_bar);
+  if (x->d_qrpub) (void)hipFree(x->d_qrpub);
   if (x->done_ev) (void)hipEventDestroy(x->done_ev);
   if (x->aux_ev) (void)hipEventDestroy(x->aux_ev);
   if (x->stream) (void)hipStreamDestroy(x->stream);

@@ -14,6 +14,8 @@
 // k_trsm_perm (one workgroup per 64-column block): solves R' Y' = Z by block back
 //   substitution on 16-row blocks (off-diagonal updates on the matrix cores) and
 //   scatters Y[perm[r]] = Y'[r].
+#include <algorithm>
+
 #include "common.h"
 
 namespace lqmc {
@@ -653,6 +655,348 @@ __global__ __launch_bounds__(QF_THREADS) void k_trsm_perm(const TrsmTask* __rest
   if (zero && st) atomicOr(&st[tk.chain].flags, 2u);
 }
 
+// ---- on-chip factorisation (N <= 256) ----------------------------------------------------------
+// k_qr_onchip: the same pre-pivoted Householder QR with the whole matrix held in registers.
+// The pre-pivoted columns are split into halves of 128, one 512-thread workgroup per half
+// (two for 128 < N <= 256, one below), and each wavefront holds one 16-column strip: 16 row
+// tiles in the 16x16x4 FP64 accumulator layout (row 16t + l/16 + 4v, column l%16 of lane l),
+// 64 doubles per lane.  Panel P is the strip of wavefront P%8 in half P/8.  A wavefront
+//   1. applies Q_P^T = I - V T^T V^T of every panel P left of its strip, from LDS, on the
+//      matrix cores: W = V^T X, W' = T^T W, X -= V W' (the accumulator layout of X is the
+//      operand layout of these products, so nothing is transposed);
+//   2. factors its own panel: the strip goes through LDS into a row-per-lane layout (rows
+//      j0 + lane + 64i), one wavefront reduction per column (Householder, no pivoting beyond
+//      the pre-pivot: the columns of the UDV chain are graded, and sorting them by norm once
+//      is what keeps the factorisation accurate), then V and T go to LDS for the others,
+//      the reflectors to A, T to Tt, |diag R| to D, and the strip's columns of D^-1 R (final:
+//      later panels touch only columns right of it) to W;
+//   3. only meets the workgroup barriers of the remaining panels (its registers are free).
+// The right half receives the left half's panels (reflectors in A, T in Tt, D) through global
+// memory behind a per-matrix release counter, hand[2 task] = panels published; the left half
+// writes W (which both halves read at the start) only after the right half has released
+// hand[2 task + 1] = 1.  The two workgroups of a matrix get block indices of equal residue
+// mod 8 (one XCD under round-robin dispatch), and a launch holds at most as many workgroups
+// as the device has compute units, so both are resident; every wait is still bounded and
+// raises flag 16 on expiry.  Outputs as k_qr_blocked (D, V'' or R' and the pivot order,
+// reflectors below the diagonal of A, T per panel in Tt), so k_qr_formq forms Q the same way.
+constexpr int QO_THREADS = 512;
+constexpr int QO_NMAX = 256;
+constexpr int QO_PS = 17;                 // LDS row stride of the strip and panel buffers
+constexpr unsigned QO_SPINS = 1u << 22;   // polls of a cross-workgroup wait
+
+typedef double qo_acc_t __attribute__((ext_vector_type(4)));
+
+__device__ inline double qo_readlane(double v, int l) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Bounded wait of one thread until *c > after (acquire at device scope); on expiry flag 16.
+__device__ inline void qo_wait(unsigned* c, unsigned after, ChainState* st, int chain, bool& broken) {
+  if (!broken) {
+    unsigned spins = 0;
+    while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= after) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins >= QO_SPINS) {
+        broken = true;
+        if (st) atomicOr(&st[chain].flags, 16u);
+        break;
+      }
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// Q_P^T applied to one strip held in registers (rows j0 = 16 P .. 16 NS - 1; the panel's V,
+// explicit with unit diagonal and zeros above it, in vbuf; T in tbuf).
+__device__ inline void qo_apply(qo_acc_t (&acc)[16], const double* vbuf, const double* tbuf, int NS, int P) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  qr_v4d wt = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    if (t < P || t >= NS) continue;
+    double a[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) a[v] = vbuf[(16 * t + lk + 4 * v) * QO_PS + li];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) wt = qr_mma(a[v], acc[t][v], wt);
+  }
+  qr_v4d w2 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < 4; ++s) w2 = qr_mma(tbuf[(4 * s + lk) * 16 + li], wt[s], w2);
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    if (t < P || t >= NS) continue;
+    double a[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) a[s] = -vbuf[(16 * t + li) * QO_PS + 4 * s + lk];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) acc[t] = qr_mma(a[s], w2[s], acc[t]);
+  }
+}
+
+struct QoShared {
+  double *sbuf, *vbuf, *tbuf, *ssc, *red, *dbuf, *taus;
+  const int* perm;
+};
+
+// Step 2 for panel P (rows j0..N-1, w columns) of the calling wavefront's strip.  hand: the
+// matrix's two hand-over counters, or null when one workgroup holds the whole matrix;
+// left: the calling workgroup is the left half of a split matrix.
+__device__ void qo_factor(qo_acc_t (&acc)[16], const QrTask& tk, const QoShared& sh, int N, int P, int w,
+                          unsigned* hand, bool left, ChainState* st, bool& broken, bool& zero_pivot) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const int j0 = P << 4;
+  double* sbuf = sh.sbuf;
+  // the whole strip to LDS; the registers are not used again
+#pragma unroll
+  for (int t = 0; t < 16; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sbuf[(16 * t + lk + 4 * v) * QO_PS + li] = acc[t][v];
+  qr_wave_sync();
+  double x[4][16];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = j0 + lane + 64 * i;
+    const bool ok = r < N;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) x[i][c] = ld0(sbuf, (size_t)r * QO_PS + c, ok);
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    if (q < w) {
+      const int j = j0 + q;
+      // s_c = x^T col_c over rows >= j for the column x = col_q (s_q = |x|^2), into red
+      {
+        double part[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) part[c] = 0.0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = j0 + lane + 64 * i;
+          const double xv = r >= j ? x[i][q] : 0.0;  // rows >= N hold zeros
+#pragma unroll
+          for (int c = q; c < 16; ++c) part[c] += xv * x[i][c];
+        }
+        qr_wave_colsum<16>(part, sh.red, lane);
+      }
+      const double alpha = qo_readlane(x[0][q], q);  // row j is lane q's first row
+      const double xx = sh.red[q];
+      const double nrmv = sqrt(xx > 0.0 ? xx : 0.0);
+      double beta, tau, scal;
+      if (nrmv == 0.0) {
+        beta = 0.0; tau = 0.0; scal = 0.0;
+        zero_pivot = true;
+      } else {
+        beta = alpha >= 0.0 ? -nrmv : nrmv;
+        tau = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+      }
+      const double ts = tau * scal;
+      double v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = j0 + lane + 64 * i;
+        v[i] = r > j ? x[i][q] * scal : (r == j ? 1.0 : 0.0);
+      }
+      // column c -= v (tau v^T col_c), tau v^T col_c = ts (s_c - beta col_c[j]), v = u / (alpha - beta)
+#pragma unroll
+      for (int c = q + 1; c < 16; ++c) {
+        const double dpc = c < w ? ts * (sh.red[c] - beta * qo_readlane(x[0][c], q)) : 0.0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) x[i][c] = x[i][c] - v[i] * dpc;
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = j0 + lane + 64 * i;
+        x[i][q] = r > j ? v[i] : (r == j ? beta : x[i][q]);
+      }
+      qr_wave_sync();  // red read by every lane before the next reduction writes it
+      if (lane == 0) {
+        sh.taus[q] = tau;
+        const double d = fabs(beta);
+        sh.dbuf[j] = d > 0.0 ? d : 1e-300;
+      }
+    }
+  }
+  // V (explicit) to vbuf, reflectors to A, R to sbuf
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = j0 + lane + 64 * i;
+    if (r >= QO_NMAX) continue;
+    const bool in = r < N;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      const int jc = j0 + c;
+      const bool col = c < w;
+      sh.vbuf[r * QO_PS + c] = (in && col) ? (r > jc ? x[i][c] : (r == jc ? 1.0 : 0.0)) : 0.0;
+      if (in && col && r <= jc) sbuf[r * QO_PS + c] = x[i][c];
+      if (in && col && r > jc) tk.A[(size_t)r * N + jc] = x[i][c];
+    }
+  }
+  qr_wave_sync();
+  qr_build_t(sh.vbuf, QO_PS, 16, j0, N, w, sh.taus, sh.tbuf, sh.ssc);
+  double* tout = tk.Tt + (size_t)P * 256;
+  for (int e = lane; e < 256; e += 64) tout[e] = ((e >> 4) < w && (e & 15) < w) ? sh.tbuf[e] : 0.0;
+  if (lane < w) tk.D[j0 + lane] = sh.dbuf[j0 + lane];
+  if (hand && left) {
+    // hand the panel (reflectors in A, T in Tt, D) to the right half
+    __builtin_amdgcn_s_waitcnt(0);
+    qr_wave_sync();
+    if (lane == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      atomicExch(hand, (unsigned)(P + 1));
+      qo_wait(hand + 1, 0u, st, tk.chain, broken);  // the right half has read W
+    }
+    qr_wave_sync();
+  }
+  // D^-1 R of the strip's columns over W: V'' = D^-1 R P^T (columns to their original
+  // places), or R' = D^-1 R with the pivot order
+  for (int e = lane; e < N * 16; e += 64) {
+    const int r = e >> 4, c = e & 15, jc = j0 + c;
+    if (c >= w) continue;
+    const double y = r <= jc ? sbuf[r * QO_PS + c] / sh.dbuf[r] : 0.0;
+    tk.W[(size_t)r * N + (tk.perm ? jc : sh.perm[jc])] = y;
+  }
+}
+
+__global__ __launch_bounds__(QO_THREADS) void k_qr_onchip(const QrTask* __restrict__ tasks, int ntasks, int G,
+                                                          unsigned* hand, ChainState* st) {
+  const int b = blockIdx.x, slot = b >> 3;
+  const int ti = (b & 7) + 8 * (slot / G), part = slot % G;
+  if (ti >= ntasks) return;
+  const QrTask tk = tasks[ti];
+  const int N = tk.N;
+  const int NS = (N + 15) >> 4;   // strips, and row tiles
+  const int Gt = (N + 127) >> 7;  // workgroups this matrix uses
+  if (part >= Gt) return;
+  unsigned* hd = Gt > 1 ? hand + 2 * ti : nullptr;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  LQMC_DYNAMIC_LDS(double, sm);
+  QoShared sh;
+  sh.sbuf = sm;                             // [256][17] strip transpose buffer, pre-pivot partials
+  sh.vbuf = sh.sbuf + QO_NMAX * QO_PS;      // [256][17] explicit V of the current panel
+  sh.tbuf = sh.vbuf + QO_NMAX * QO_PS;      // [16][16] T of the current panel
+  sh.ssc = sh.tbuf + 256;                   // [16][16] S = V^T V scratch
+  sh.red = sh.ssc + 256;                    // [16] wavefront column sums
+  sh.dbuf = sh.red + 16;                    // [256] |diag R|
+  sh.taus = sh.dbuf + QO_NMAX;              // [16] reflector coefficients of the panel
+  double* norms = sh.taus + 16;             // [256] pre-pivot column norms
+  int* perm = reinterpret_cast<int*>(norms + QO_NMAX);  // [256] pre-pivot order
+  sh.perm = perm;
+  bool zero_pivot = false, bad = false, broken = false;
+
+  // pre-pivoting: column norms (partial sums over row groups, combined in group order, so
+  // both workgroups of a matrix derive the same order), then decreasing norm, ties by index,
+  // NaN last
+  {
+    const int ng = QO_THREADS / N;  // >= 2 row groups
+    if (tid < ng * N) {
+      const int c = tid % N, g = tid / N;
+      double s = 0.0;
+      for (int r0 = g; r0 < N; r0 += 8 * ng) {
+        double xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int r = r0 + u * ng;
+          xv[u] = ldg0(gptr(tk.W), (size_t)r * N + c, r < N);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += xv[u] * xv[u];
+      }
+      sh.sbuf[g * N + c] = s;
+    }
+    __syncthreads();
+    for (int c = tid; c < N; c += QO_THREADS) {
+      double s = 0.0;
+      for (int g = 0; g < ng; ++g) s += sh.sbuf[g * N + c];
+      if (!(s >= 0.0)) { s = -1.0; bad = true; }
+      norms[c] = s;
+    }
+    for (int i = tid; i < QO_NMAX; i += QO_THREADS) sh.dbuf[i] = 1.0;
+    __syncthreads();
+    for (int c = tid; c < N; c += QO_THREADS) {
+      const double nc = norms[c];
+      int rank = 0;
+      for (int c2 = 0; c2 < N; ++c2) {
+        const double n2 = norms[c2];
+        rank += (n2 > nc) || (n2 == nc && c2 < c);
+      }
+      perm[rank] = c;
+    }
+    __syncthreads();
+  }
+  // my strip
+  const int S = part * 8 + wave;
+  const bool have = S < NS;
+  qo_acc_t acc[16];
+  {
+    const int c = 16 * S + li;
+    const bool cok = have && c < N;
+    const int pc = perm[cok ? c : 0];
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = 16 * t + lk + 4 * v;
+        acc[t][v] = ldg0(gptr(tk.W), (size_t)r * N + pc, cok && r < N);
+      }
+  }
+  // every read of W is complete before any workgroup of the matrix writes it
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (hd && part == 1 && tid == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    atomicExch(hd + 1, 1u);
+  }
+  const int P0 = part * 8, P1 = min(NS, P0 + 8);  // the panels of my half
+  const int Sm = have ? S : P1;
+  // 1. panels left of my strip (received from the left half, or factored by a wavefront of
+  //    my half before the first barrier of their iteration)
+  for (int P = 0; P < Sm; ++P) {
+    if (P < P0) {
+      const int j0 = P << 4, w = min(16, N - j0);
+      if (tid == 0) qo_wait(hd, (unsigned)P, st, tk.chain, broken);
+      __syncthreads();
+      for (int e = tid; e < (QO_NMAX - j0) * 16; e += QO_THREADS) {
+        const int r = j0 + (e >> 4), c = e & 15, jc = j0 + c;
+        const bool in = r < N && c < w;
+        const double a = ldg0(gptr(tk.A), (size_t)r * N + jc, in && r > jc);
+        sh.vbuf[r * QO_PS + c] = in ? (r > jc ? a : (r == jc ? 1.0 : 0.0)) : 0.0;
+      }
+      for (int e = tid; e < 256; e += QO_THREADS) sh.tbuf[e] = tk.Tt[(size_t)P * 256 + e];
+      if (tid < w) sh.dbuf[j0 + tid] = tk.D[j0 + tid];
+    }
+    __syncthreads();
+    if (have) qo_apply(acc, sh.vbuf, sh.tbuf, NS, P);
+    __syncthreads();
+  }
+  // 2. my panel, 3. the panels right of it
+  if (have) {
+    qo_factor(acc, tk, sh, N, S, min(16, N - 16 * S), hd, part == 0, st, broken, zero_pivot);
+    for (int P = S; P < P1; ++P) {
+      __syncthreads();
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  if (hd && tid == 0) atomicExch(hd + (part == 1 ? 0 : 1), 0u);  // counters back to zero for the next launch
+  if (part == 0 && tk.perm)
+    for (int i = tid; i < N; i += QO_THREADS) tk.perm[i] = perm[i];
+  if (st) {
+    if (zero_pivot && lane == 0) atomicOr(&st[tk.chain].flags, 2u);
+    if (bad) atomicOr(&st[tk.chain].flags, 8u);
+  }
+}
+
+size_t qr_onchip_shmem() {
+  return sizeof(double) * (2 * (size_t)QO_NMAX * QO_PS + 256 + 256 + 16 + QO_NMAX + 16 + QO_NMAX) +
+         sizeof(int) * QO_NMAX;
+}
+
 size_t qr_blocked_shmem(int nmax) {
   const size_t nb = nmax > 256 ? 8 : 16;
   return sizeof(double) * (nmax * (nb + 1) + nb * nb + 256 + 48 + 2 * (size_t)nmax) + sizeof(int) * (nmax + 33);
@@ -665,9 +1009,29 @@ size_t qr_formq_shmem(int nmax) {
 
 size_t trsm_shmem(int) { return sizeof(double) * (TRSM_B * QF_COLS + TRSM_B * TRSM_B); }
 
-void launch_qr_blocked(const QrTask* d, int n, int nmax, ChainState* st, hipStream_t s) {
+// QR of n matrices (explicit Q formed).  N <= 256: the on-chip factorisation, at most
+// max_wg workgroups per launch (the compute units, so the two workgroups of a matrix are
+// resident together) with hand[2 n] (zero between launches) for the hand-over between them;
+// larger N: the blocked factorisation with the trailing matrix in global memory.
+void launch_qr_blocked(const QrTask* d, int n, int nmax, ChainState* st, unsigned* hand, int max_wg, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_qr_blocked, dim3(n), dim3(QR_THREADS), qr_blocked_shmem(nmax), s, d, st);
+  if (nmax <= QO_NMAX) {
+    static bool attr = false;
+    if (!attr) {
+      (void)hipFuncSetAttribute((const void*)k_qr_onchip, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)qr_onchip_shmem());
+      attr = true;
+    }
+    const int G = (nmax + 127) >> 7;
+    const int chunk = G > 1 ? std::max(8, (max_wg / G) / 8 * 8) : n;
+    for (int c0 = 0; c0 < n; c0 += chunk) {
+      const int cnt = std::min(chunk, n - c0);
+      const int grid = 8 * G * ((cnt + 7) / 8);
+      hipLaunchKernelGGL(k_qr_onchip, dim3(grid), dim3(QO_THREADS), qr_onchip_shmem(), s, d + c0, cnt, G, hand, st);
+    }
+  } else {
+    hipLaunchKernelGGL(k_qr_blocked, dim3(n), dim3(QR_THREADS), qr_blocked_shmem(nmax), s, d, st);
+  }
   const int nblk = (nmax + QF_COLS - 1) / QF_COLS;
   hipLaunchKernelGGL(k_qr_formq, dim3(n * nblk), dim3(QF_THREADS), qr_formq_shmem(nmax), s, d, nblk);
 }

@@ -90,6 +90,7 @@ inline void emu_wave_sync() { emu_waves[threadIdx.x >> 6].bar->arrive_and_wait()
 #define __builtin_amdgcn_wave_barrier() emu_wave_sync()
 #define __builtin_amdgcn_fence(order, scope) ((void)0)
 #define __builtin_amdgcn_sched_barrier(mask) ((void)0)
+#define __builtin_amdgcn_s_waitcnt(mask) ((void)0)
 
 // Cross-lane exchange within a wavefront.
 inline double __shfl_xor(double v, int mask, int width = 64) {
@@ -174,6 +175,30 @@ inline void emu_launch(int blocks, int threads, size_t lds_bytes, F fn) {
   blockDim.x = threads;
   gridDim.x = blocks;
   for (int b = 0; b < blocks; ++b) emu_run_blocks(b, b + 1, threads, lds_bytes, fn);
+}
+
+// The blocks with the given indices, all at the same time (a grid whose other blocks return
+// at once).
+template <typename F>
+inline void emu_launch_ids(const std::vector<int>& ids, int grid, int threads, size_t lds_bytes, F fn) {
+  blockDim.x = threads;
+  gridDim.x = grid;
+  std::vector<EmuBlock*> blk;
+  for (size_t k = 0; k < ids.size(); ++k) blk.push_back(new EmuBlock(threads, lds_bytes));
+  std::vector<std::thread> ts;
+  for (size_t k = 0; k < ids.size(); ++k)
+    for (int t = 0; t < threads; ++t)
+      ts.emplace_back([&, t, k] {
+        EmuBlock* e = blk[k];
+        threadIdx.x = t;
+        blockIdx.x = ids[k];
+        emu_lds_buf = e->lds;
+        emu_block_barrier = &e->bar;
+        emu_waves = e->waves.data();
+        fn();
+      });
+  for (auto& t : ts) t.join();
+  for (auto* e : blk) delete e;
 }
 
 // As emu_launch, with every block running at the same time.

@@ -2,7 +2,9 @@
 // CPU build: graded test matrices W = M diag(d) (d over many decades, like the
 // UDV chain), checks W = Q diag(D) V'' (no pivot request), W P = Q diag(D) R' (pivot
 // order requested), Q^T Q = I and the permuted triangular solve.
-// usage: qr_harness N decades seed [near]
+// usage: qr_harness N decades seed [near|onchip]
+// onchip: the factorisation by k_qr_onchip (N <= 256: one workgroup per 128 columns, both at
+// once) instead of k_qr_blocked; its hand-over counters must be back at zero afterwards.
 #include "hip_cpu_shim.h"
 #include "../../latticeqmc_amd/csrc/qr.hip"
 
@@ -27,6 +29,18 @@ int main(int argc, char** argv) {
   // "near": columns 2i+1 = columns 2i + 1e-9 noise at the same scale, so a panel that takes one
   // of a pair as pivot leaves the other with a tiny remaining norm (the recompute safeguard)
   const bool near = argc > 4 && std::string(argv[4]) == "near";
+  const bool onchip = argc > 4 && std::string(argv[4]) == "onchip";
+  unsigned hand[2] = {0u, 0u};
+  auto factor = [&](QrTask* task, ChainState* stp) {
+    if (onchip) {
+      const int G = (N + 127) / 128;
+      std::vector<int> ids{0};
+      if (G > 1) ids.push_back(8);
+      emu_launch_ids(ids, 8 * G, QO_THREADS, qr_onchip_shmem(), [&] { k_qr_onchip(task, 1, G, hand, stp); });
+    } else {
+      emu_launch(1, QR_THREADS, qr_blocked_shmem(N), [&] { k_qr_blocked(task, stp); });
+    }
+  };
   if (near)
     for (int c = 1; c < N; c += 2) {
       d[c] = d[c - 1];
@@ -45,7 +59,7 @@ int main(int argc, char** argv) {
   // pass 1 without a pivot request: W <- V'' = D^-1 R P^T, W0 = Q diag(D) V''
   std::vector<double> Wv(W0);
   QrTask qv{Wv.data(), Q, D, A, nullptr, Tt, N, 0};
-  emu_launch(1, QR_THREADS, qr_blocked_shmem(N), [&] { k_qr_blocked(&qv, &st); });
+  factor(&qv, &st);
   emu_launch(nblk, QF_THREADS, qr_formq_shmem(N), [&] { k_qr_formq(&qv, nblk); });
   double recv = 0.0;
   for (int c = 0; c < N; ++c) {
@@ -62,7 +76,7 @@ int main(int argc, char** argv) {
   // pass 2 with the pivot order: W <- R' = D^-1 R, W0[:, perm[c]] = Q diag(D) R'[:, c]
   std::fill(Q, Q + n2, 0.0);
   QrTask qt{W, Q, D, A, perm, Tt, N, 0};
-  emu_launch(1, QR_THREADS, qr_blocked_shmem(N), [&] { k_qr_blocked(&qt, &st); });
+  factor(&qt, &st);
   emu_launch(nblk, QF_THREADS, qr_formq_shmem(N), [&] { k_qr_formq(&qt, nblk); });
   // reconstruction, column-relative: |W - Q D R'|_col / |W|_col
   double rec = 0.0, orth = 0.0;
@@ -118,7 +132,7 @@ int main(int argc, char** argv) {
   // the solve is held to its backward error instead of the residual and the |R'| bound
   const bool solve_ok = near ? bwd < 1e-15 : (vmax < 1e3 && res / zn < 1e-9);
   const bool ok = recv < 1e-12 && rec < 1e-12 && orth < 1e-12 && low == 0.0 && dia < 1e-12 && solve_ok &&
-                  st.flags == 0;
+                  st.flags == 0 && hand[0] == 0u && hand[1] == 0u;
   std::printf(ok ? "ok\n" : "FAIL\n");
   return ok ? 0 : 1;
 }

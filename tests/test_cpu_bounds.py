@@ -92,19 +92,23 @@ def qr_harness_tsan(tmp_path_factory):
     return _build_qr(tmp_path_factory, "thread")
 
 
-@pytest.mark.parametrize("args", ["4 2 1", "37 20 3", "100 40 5", "257 40 6", "64 20 7 near"])
+@pytest.mark.parametrize("args", ["4 2 1", "37 20 3", "100 40 5", "257 40 6", "64 20 7 near",
+                                  "37 20 3 onchip", "100 40 5 onchip", "128 30 2 onchip", "129 30 4 onchip",
+                                  "256 40 6 onchip"])
 def test_blocked_qr_and_trsm(qr_harness, args):
     """QR, the column-block Q formation and the permuted triangular solve on graded
     matrices: W = Q diag(D) V'' and W P = Q diag(D) R' to 1e-12 column-relative, Q
     orthogonal, R' upper triangular with unit-modulus diagonal, R' Y' = Z to 1e-9,
     all accesses in bounds.  "near" pairs nearly dependent columns, so the panel's
-    downdated column norms collapse and take the exact-recompute path."""
+    downdated column norms collapse and take the exact-recompute path.  "onchip" factors
+    with the register-resident kernel (N <= 256; above 128 columns two workgroups run at
+    once and hand the left half's panels over through memory)."""
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
     r = subprocess.run([qr_harness, *args.split()], capture_output=True, text=True, env=env, timeout=900)
     assert r.returncode == 0 and "ok" in r.stdout, (r.stdout + r.stderr)[-3000:]
 
 
-@pytest.mark.parametrize("args", ["70 20 2"])
+@pytest.mark.parametrize("args", ["70 20 2", "70 20 2 onchip", "150 30 3 onchip"])
 def test_blocked_qr_race_free(qr_harness_tsan, args):
     """No cross-wavefront race in the QR, Q-formation and solve kernels.  (The 8-column
     panels of N > 256 are factored by one wavefront; their multi-wavefront phases are the
