@@ -111,26 +111,51 @@ def time_form(specs, steps, warmup, profile_kind=None, **kw):
     return dict(ms_per_step=1e3 * dt / steps, ups=upd * steps / dt, stats=st, prof=prof, upd_per_step=upd)
 
 
-def cpu_baseline(cfg, budget_s=15.0, procs=None):
-    """Reference-speed CPU sweep (faithful restatement, the reference's j,k loop and
-    per-attempt mean/var) on a bounded sample, one chain per process like the
-    reference's chain pool (lqmc/multiprocessing.py:252-289)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+# Bounded samples per process (whole slices of one chain of the workload, at the reference's
+# ~20 upd/s per core the faithful form needs ~13 s for one 16x16 slice).
+CPU_SAMPLES = {"faithful": 1, "outer": 16, "dqmc": 8}
+
+
+def cpu_baseline(cfg, procs=None):
+    """CPU sweeps of the workload on the host cores, one chain per process like the
+    reference's chain pool (lqmc/multiprocessing.py:252-289), each on a bounded sample of
+    whole slices, with one BLAS thread per process:
+      faithful  lqmc.py:301-347 as written -- the pure-Python j,k update loop and the
+                per-attempt mean/var of _debug (lqmc.py:217-232); the headline baseline;
+      outer     the same sweep with the rank-1 update as np.outer;
+      dqmc      the oracle's stabilised textbook sweep (the physics the GPU form computes).
+    The pool holds min(16, cpu_count) processes: the host share of one GPU on the
+    benchmark machines is 16 cores."""
     import multiprocessing as mp
-    procs = procs or min(16, os.cpu_count() or 1)
+    ncpu = os.cpu_count() or 1
+    procs = procs or min(16, ncpu)
     n = cfg["sides"][0] ** 2
-    # one slice of a 16x16 chain is ~256 attempts at ~20 upd/s per core in the faithful form
-    slices = 1
     ctx = mp.get_context("fork")
+    out = {}
     with ctx.Pool(procs, initializer=_pool_init) as pool:
-        t0 = time.perf_counter()
-        counts = pool.map(_cpu_worker, [(cfg["sides"][c % len(cfg["sides"])], cfg["u"][0], cfg["beta"][0],
-                                         cfg["L"][0], c, slices) for c in range(procs)])
-        dt = time.perf_counter() - t0
-    total = sum(counts)
-    return dict(value=total / dt, unit="spin-flip updates/s", cores=procs, kind="port",
+        for kind, slices in CPU_SAMPLES.items():
+            t0 = time.perf_counter()
+            counts = pool.map(_cpu_worker, [(kind, cfg["sides"][c % len(cfg["sides"])], cfg["u"][0], cfg["beta"][0],
+                                             cfg["L"][0], c, slices) for c in range(procs)])
+            dt = time.perf_counter() - t0
+            out[kind] = dict(value=sum(counts) / dt, attempts=sum(counts), seconds=dt, slices_per_process=slices)
+    f = out["faithful"]
+    return dict(value=f["value"], unit="spin-flip updates/s", cores=procs, kind="port",
                 sample=f"faithful restatement of lqmc.py:301-347 (Python j,k loop, per-attempt mean/var), "
-                       f"{slices} slice x {n} sites per process incl. the sweep-start inversion, "
-                       f"{procs} processes, {total} attempts in {dt:.1f} s")
+                       f"{CPU_SAMPLES['faithful']} slice x {n} sites per process incl. the sweep-start inversion, "
+                       f"{procs} processes, {f['attempts']} attempts in {f['seconds']:.1f} s",
+                cpu_model=cpu_model(), cpu_count=ncpu, variants=out)
 
 
 def _pool_init():
@@ -143,13 +168,50 @@ def _pool_init():
 
 
 def _cpu_worker(args):
-    side, u, beta, L, seed, slices = args
-    from oracle import mode_r, model
+    kind, side, u, beta, L, seed, slices = args
+    from oracle import dqmc, mode_r, model
     k = model.square_k(side, side)
     _, lam, ek, _ = model.params(k, beta, L, u)
     f = model.random_field(side * side, L, seed)
     uni = np.random.RandomState(seed + 1000).rand(side * side * L)
-    return mode_r.sweep_faithful(f, ek, lam, uni, max_slices=slices)
+    if kind == "faithful":
+        return mode_r.sweep_faithful(f, ek, lam, uni, max_slices=slices)
+    if kind == "outer":
+        mode_r.sweep(f, ek, lam, uni, max_slices=slices)
+        return slices * side * side
+    o = dqmc.DQMCOracle(k, beta, L, u, f, n_stab=8)
+    o.sweep(uni, max_slices=slices)
+    return slices * side * side
+
+
+def step_roofline(specs, n_stab, delay, esz, acc, ms_per_step):
+    """Whole-sweep roofline of the stabilised delayed form: T_roof = F64/P64 + F32/P32 +
+    bytes/BW with the algorithmic work per chain and spin
+      left stack   (L - n_s) slice GEMMs 2N^3 + (nb - 1) (QR 8/3 N^3 incl. Q, V update 2N^3)
+      wraps        L x 2 GEMMs (4N^3)            -- in the form's precision
+      right chain  L GEMMs 2N^3 + nb (QR 8/3 N^3 + V update 2N^3)
+      recompute    nb x (X1 2N^3 + X2 2N^3 + QR(X) 8/3 N^3 + Z 2N^3 + solve N^3 + G 2N^3)
+      flips        L x 2aN^3 (the rank-k products)        -- in the form's precision
+    and the flips' G traffic, 4 N^2 s / k bytes per attempted update (both spins' G read and
+    written once per block of k sites).  Diagonal scalings, ratios and copies are not counted."""
+    f64 = f_form = byts = 0.0
+    for sp in specs:
+        n, L = float(sp.n_sites), sp.time_steps
+        n3 = n ** 3
+        nb = -(-L // n_stab)
+        stab = ((L - n_stab) * 2 + (nb - 1) * (8 / 3 + 2) + L * 2 + nb * (8 / 3 + 2)
+                + nb * (2 + 2 + 8 / 3 + 2 + 1 + 2)) * n3
+        form = (L * 4 + L * 2 * acc) * n3
+        f64 += 2 * stab
+        f_form += 2 * form
+        byts += 4 * n * n * esz / max(delay, 1) * n * L if delay > 1 else 4 * acc * n * n * esz * n * L
+    p_form = PEAK["fp32"] if esz == 4 else PEAK["fp64"]
+    if esz == 8:
+        f64, f_form = f64 + f_form, 0.0
+    t = (f64 / (PEAK["fp64"] * 1e12) + f_form / (p_form * 1e12) + byts / (PEAK["hbm"] * 1e9)) * 1e3
+    return {"t_roof_ms": t, "ms_per_step": ms_per_step, "frac": t / ms_per_step, "flops_fp64": f64,
+            "flops_form": f_form, "flip_bytes": byts,
+            "model": "T = F64/78.6 TF + F_form/P_form + flip bytes/8 TB/s, algorithmic counts per sweep"}
 
 
 def log(msg):
@@ -219,7 +281,7 @@ def main():
     log(f"[bench] chosen: {chosen['ms_per_step']:.1f} ms/sweep, max|dG| {chosen['stats']['max_err']:.2e}")
     plain = None
     if not args.no_plain:
-        plain = time_form(specs, args.steps, args.warmup, profile_kind=None, mode="dqmc", precision="fp64",
+        plain = time_form(specs, args.steps, args.warmup, profile_kind="flip", mode="dqmc", precision="fp64",
                           delay=1, n_stab=args.n_stab, graph=False)
         log(f"[bench] plain: {plain['ms_per_step']:.1f} ms/sweep, max|dG| {plain['stats']['max_err']:.2e}")
     fp32 = None
@@ -250,11 +312,20 @@ def main():
         "gate_1e-6": bool(st["max_err"] <= 1e-6),
         "acceptance": st["accepted"] / max(st["attempted"], 1),
     }
+    esz = 4 if args.precision == "fp32" else 8
+    acc = st["accepted"] / max(st["attempted"], 1)
+    out["step_roofline"] = step_roofline(specs, args.n_stab, args.delay, esz, acc, chosen["ms_per_step"])
     if plain:
         out["plain_fp64_value"] = plain["ups"]
         out["plain_ms_per_step"] = plain["ms_per_step"]
         out["plain_max_dG"] = plain["stats"]["max_err"]
         out["speedup_vs_plain"] = chosen["ups"] / plain["ups"]
+        if plain.get("prof"):
+            pargs = argparse.Namespace(**vars(args))
+            pargs.delay, pargs.precision = 1, "fp64"
+            out["plain_roofline"] = roofline("flip", *plain["prof"], specs, pargs, plain)
+        out["plain_step_roofline"] = step_roofline(specs, args.n_stab, 1, 8, plain["stats"]["accepted"] /
+                                                   max(plain["stats"]["attempted"], 1), plain["ms_per_step"])
     if fp32:
         out["fp32_form"] = {"value": fp32["ups"], "max_dG_vs_fp64": fp32["stats"]["max_err"],
                             "gate_1e-6": bool(fp32["stats"]["max_err"] <= 1e-6)}

@@ -58,6 +58,15 @@ def load(path=None):
     if not os.path.exists(path):
         raise ImportError(f"{path} is missing: build it with `python -m latticeqmc_amd.build` "
                           "(the GPU sweep has no CPU fallback)")
+    # One HIP runtime per process.  PyTorch-ROCm ships its own libamdhip64; once torch is
+    # imported, the library's HIP symbols bind to that copy, so torch and the engine share one
+    # runtime (device pointers and stream handles pass between them).  Loaded before torch, the
+    # library would bind to the system runtime instead, and whichever of the two runtimes
+    # initialises the GPU second finds no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     P, I, D, V = ctypes.POINTER, ctypes.c_int, ctypes.c_double, ctypes.c_void_p
     i32p, f64p = P(ctypes.c_int32), P(ctypes.c_double)

@@ -211,8 +211,9 @@ class DQMCOracle:
             self.g[s_i] = self.expk_inv @ gs @ self.expk
 
     # -- sweep -------------------------------------------------------------------------------
-    def sweep(self, uniforms):
-        """One sweep; returns dict(accept, ratio, g_meas, errors)."""
+    def sweep(self, uniforms, max_slices=None):
+        """One sweep; returns dict(accept, ratio, g_meas, errors).  ``max_slices`` stops after
+        that many slices (a bounded sample for timing; the chain is then mid-sweep)."""
         L, n = self.L, self.n
         uniforms = np.asarray(uniforms, dtype=np.float64).reshape(L, n)
         stacks = [self.left_stack(+1, 1), self.left_stack(-1, 1)]
@@ -223,6 +224,8 @@ class DQMCOracle:
         errs = []
         gtd = []  # (l, [G_up(l dtau, 0), G_dn(l dtau, 0)]) at each stabilisation point
         for t in range(L):
+            if max_slices is not None and t >= max_slices:
+                break
             l = L - 1 - t
             if self.delay > 1:
                 self.slice_delayed(l, uniforms[t], rec)

@@ -39,8 +39,9 @@ def get_m(field, expk, lam, l0, sigma):
     return np.eye(field.shape[0]) + b_prod
 
 
-def sweep(field, expk, lam, uniforms):
-    """One Mode R sweep; mutates ``field``. Returns (g_up, g_dn, ratios, accepts)."""
+def sweep(field, expk, lam, uniforms, max_slices=None):
+    """One Mode R sweep; mutates ``field``. Returns (g_up, g_dn, ratios, accepts).
+    ``max_slices`` stops after that many slices (a bounded sample for timing)."""
     n, L = field.shape
     g_up = np.linalg.inv(get_m(field, expk, lam, 0, +1))
     g_dn = np.linalg.inv(get_m(field, expk, lam, 0, -1))
@@ -48,6 +49,8 @@ def sweep(field, expk, lam, uniforms):
     accs = np.zeros(n * L, dtype=bool)
     q = 0
     for l in reversed(range(L)):
+        if max_slices is not None and L - 1 - l >= max_slices:
+            break
         for i in range(n):
             arg = 2 * lam * field[i, l]
             d_up = 1 + (1 - g_up[i, i]) * (np.exp(+arg) - 1)
