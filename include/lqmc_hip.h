@@ -197,6 +197,17 @@ size_t lqmc_wrap_green_workspace(int n, int count);
 int lqmc_wrap_green(int device, int n, int count, const double* expk, const double* expk_inv, const double* v,
                     double* g, void* workspace, size_t ws_bytes, void* stream);
 
+/* lqmc_wrap_green without the workspace argument: the same kernels on a static device
+ * workspace of 16 MiB per GPU (allocated with the library's code object, held until the
+ * process ends), so the call allocates nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Limit: lqmc_wrap_green_workspace(n, count) <=
+ * 16 MiB (e.g. count <= 31 at n = 256); larger calls return LQMC_EINVAL.  All calls on one
+ * GPU share its workspace: calls on one stream are ordered, calls that overlap on one GPU
+ * (other streams or threads) race. */
+int lqmc_wrap_green_static(int device, int n, int count, const double* expk, const double* expk_inv,
+                           const double* v, double* g, void* stream);
+
 const char* lqmc_last_error(void);
 int lqmc_destroy(lqmc_ctx* ctx);
 

@@ -23,7 +23,7 @@ EXPORTS = (
     "lqmc_get_field",
     "lqmc_get_stats", "lqmc_reset_stats", "lqmc_get_trace", "lqmc_get_observables",
     "lqmc_profile", "lqmc_profile_result", "lqmc_wrap_green_workspace", "lqmc_wrap_green",
-    "lqmc_last_error", "lqmc_destroy",
+    "lqmc_wrap_green_static", "lqmc_last_error", "lqmc_destroy",
 )
 
 
@@ -92,6 +92,7 @@ def load(path=None):
         "lqmc_profile_result": (I, [V, f64p, P(ctypes.c_int64), f64p]),
         "lqmc_wrap_green_workspace": (ctypes.c_size_t, [I, I]),
         "lqmc_wrap_green": (I, [I, I, I, V, V, V, V, V, ctypes.c_size_t, V]),
+        "lqmc_wrap_green_static": (I, [I, I, I, V, V, V, V, V]),
         "lqmc_last_error": (ctypes.c_char_p, []),
         "lqmc_destroy": (I, [V]),
     }

@@ -8,6 +8,7 @@
 // descriptors are written on the device by a setup kernel from the launch arguments, so
 // a call copies nothing from the host and can be captured into a graph.
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/lqmc_hip.h"
@@ -89,6 +90,36 @@ int wrap_issue(int n, int count, const double* expk, const double* expk_inv, con
   return LQMC_OK;
 }
 
+// ---- allocation-free variant ------------------------------------------------------------------
+// lqmc_wrap_green_static keeps the workspace of lqmc_wrap_green in a file-scope device array of
+// WRAP_SCRATCH_BYTES, so a call allocates nothing (for code under graph capture, serving loops
+// and harnesses that forbid hipMalloc).  Each GPU holds its own copy of the array until the
+// process ends.  In host code the array's name is not a device address: the address of each
+// device's copy is looked up once per device with hipGetSymbolAddress and kept in
+// g_scratch_ptr, and only that pointer is handed to the kernels.  Every call on a GPU shares
+// that GPU's array: calls on one stream are ordered, calls that overlap on one GPU (other
+// streams or threads) race.  Nothing in it needs clearing: each call writes its descriptors
+// and every temporary entry before reading them.
+constexpr size_t WRAP_SCRATCH_BYTES = size_t(16) << 20;
+constexpr int WRAP_MAX_DEVICES = 64;
+static __device__ __align__(256) char scratch[WRAP_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static char* g_scratch_ptr[WRAP_MAX_DEVICES] = {};
+
+// Device address of `scratch` on `device` (current device on return); null on failure.
+char* scratch_on(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return g_scratch_ptr[device];
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    lqmc_set_error(std::string("lqmc_wrap_green_static: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+    return nullptr;
+  }
+  g_scratch_ptr[device] = static_cast<char*>(p);
+  return g_scratch_ptr[device];
+}
+
 }  // namespace
 
 extern "C" {
@@ -111,6 +142,29 @@ int lqmc_wrap_green(int device, int n, int count, const double* expk, const doub
     return LQMC_EHIP;
   }
   return wrap_issue(n, count, expk, expk_inv, v, g, static_cast<char*>(workspace), static_cast<hipStream_t>(stream));
+}
+
+int lqmc_wrap_green_static(int device, int n, int count, const double* expk, const double* expk_inv,
+                           const double* v, double* g, void* stream) {
+  int rc = wrap_check(n, count, expk, expk_inv, v, g);
+  if (rc) return rc;
+  if (wrap_bytes(n, count) > WRAP_SCRATCH_BYTES) {
+    lqmc_set_error("lqmc_wrap_green_static: n*n*count*8 bytes plus descriptors exceed the 16 MiB static "
+                   "workspace; use lqmc_wrap_green with a workspace of lqmc_wrap_green_workspace(n, count) bytes");
+    return LQMC_EINVAL;
+  }
+  if (device < 0 || device >= WRAP_MAX_DEVICES) {
+    lqmc_set_error("lqmc_wrap_green_static: device index out of range");
+    return LQMC_EINVAL;
+  }
+  const hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    lqmc_set_error(std::string("lqmc_wrap_green_static: hipSetDevice failed: ") + hipGetErrorString(e));
+    return LQMC_EHIP;
+  }
+  char* ws = scratch_on(device);
+  if (!ws) return LQMC_EHIP;
+  return wrap_issue(n, count, expk, expk_inv, v, g, ws, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"
