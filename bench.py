@@ -99,6 +99,8 @@ def time_form(specs, steps, warmup, profile_kind=None, **kw):
     sync_all()
     dt = time.perf_counter() - t0
     st = eng.stats()
+    rec = np.max([eng.recompute_errors(i) for i in range(len(specs))], axis=0).tolist() if len(
+        {s.time_steps for s in specs}) == 1 else None
     prof = None
     if profile_kind:
         eng.use_graph(False)
@@ -108,7 +110,7 @@ def time_form(specs, steps, warmup, profile_kind=None, **kw):
         prof = eng.profile_result()
         eng.profile("off")
     eng.close()
-    return dict(ms_per_step=1e3 * dt / steps, ups=upd * steps / dt, stats=st, prof=prof, upd_per_step=upd)
+    return dict(ms_per_step=1e3 * dt / steps, ups=upd * steps / dt, stats=st, prof=prof, upd_per_step=upd, rec=rec)
 
 
 def cpu_model():
@@ -228,7 +230,9 @@ def main():
     ap.add_argument("--n-stab", type=int, default=0, help="0: autotuned/cached choice")
     ap.add_argument("--autotune", action="store_true", help="tune (k, n_s) under the 1e-6 gate first")
     ap.add_argument("--tune-cache", default=None, help="autotune JSON cache path")
-    ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--precision", default="fp64", help="fp64 | fp32 | bf16x3 (FP32 G, split-BF16 rank-k products)")
+    ap.add_argument("--max-dg", type=float, default=0.0,
+                    help="recompute deviation that raises the device precision flag (0: 1e-3; negative: off)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
@@ -277,7 +281,8 @@ def main():
         args.n_stab = entry["n_stab"] if tuned else dflt[1]
     log(f"[bench] chosen form: {args.precision} delay={args.delay} n_stab={args.n_stab} graph={not args.no_graph}")
     chosen = time_form(specs, args.steps, args.warmup, profile_kind=args.profile_kind, mode="dqmc",
-                       precision=args.precision, delay=args.delay, n_stab=args.n_stab, graph=not args.no_graph)
+                       precision=args.precision, delay=args.delay, n_stab=args.n_stab, graph=not args.no_graph,
+                       max_dg=args.max_dg)
     log(f"[bench] chosen: {chosen['ms_per_step']:.1f} ms/sweep, max|dG| {chosen['stats']['max_err']:.2e}")
     plain = None
     if not args.no_plain:
@@ -309,10 +314,11 @@ def main():
                            + ("hipGraph" if not args.no_graph else "no graph"),
                    "delay": args.delay, "n_stab": args.n_stab, "autotuned": tuned},
         "max_dG_vs_fp64": st["max_err"],
+        "max_dG_per_recompute": chosen.get("rec"),
         "gate_1e-6": bool(st["max_err"] <= 1e-6),
         "acceptance": st["accepted"] / max(st["attempted"], 1),
     }
-    esz = 4 if args.precision == "fp32" else 8
+    esz = 8 if args.precision == "fp64" else 4
     acc = st["accepted"] / max(st["attempted"], 1)
     out["step_roofline"] = step_roofline(specs, args.n_stab, args.delay, esz, acc, chosen["ms_per_step"])
     if plain:
@@ -362,8 +368,8 @@ def roofline(kind, avg_ms, launches, work, specs, args, form):
     st = form["stats"]
     acc = st["accepted"] / max(st["attempted"], 1)
     n = specs[0].n_sites
-    esz = 4 if args.precision == "fp32" else 8
-    peak_mfma = PEAK["fp32"] if args.precision == "fp32" else PEAK["fp64"]
+    esz = 8 if args.precision == "fp64" else 4
+    peak_mfma = PEAK["fp64"] if args.precision == "fp64" else PEAK["fp32"]
     if kind == "flip":
         flips_launch = work / max(launches, 1)
         if args.delay > 1:

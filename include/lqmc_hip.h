@@ -41,7 +41,7 @@ enum {
 };
 
 enum { LQMC_MODE_DQMC = 0, LQMC_MODE_REFERENCE = 1, LQMC_MODE_DET = 2 };
-enum { LQMC_FP64 = 0, LQMC_FP32 = 1 };
+enum { LQMC_FP64 = 0, LQMC_FP32 = 1, LQMC_BF16X3 = 2 };
 enum { LQMC_RNG_REPLAY = 0, LQMC_RNG_PHILOX = 1 };
 
 /* Device-side error flag bits reported in lqmc_stats.flags. */
@@ -58,7 +58,11 @@ typedef struct {
   int32_t mode;        /* LQMC_MODE_DQMC (textbook, stabilised), LQMC_MODE_REFERENCE, or
                           LQMC_MODE_DET: the reference's brute-force determinant sweep
                           (lqmc/lqmc.py:236-299), FP64 rank-1 only, N <= 32 */
-  int32_t precision;   /* LQMC_FP64 or LQMC_FP32: storage of G, wraps and flip updates */
+  int32_t precision;   /* LQMC_FP64 or LQMC_FP32: storage of G, wraps and flip updates;
+                          LQMC_BF16X3: the FP32 form with the delayed rank-k products G += P Z
+                          as three BF16 matrix-core products with FP32 accumulation
+                          (P_hi Z_hi + P_hi Z_lo + P_lo Z_hi, x = x_hi + x_lo split in BF16),
+                          Mode D, delay >= 4 */
   int32_t delay;       /* 1: rank-1 update per accepted flip; k>1: submatrix blocks of k sites */
   int32_t n_stab;      /* slices between FP64 QR/UDV recomputes (Mode D) */
   int32_t rng;         /* LQMC_RNG_REPLAY or LQMC_RNG_PHILOX */

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblqmc_hip.so")
 
 MODE_DQMC, MODE_REFERENCE, MODE_DET = 0, 1, 2
-FP64, FP32 = 0, 1
+FP64, FP32, BF16X3 = 0, 1, 2
 RNG_REPLAY, RNG_PHILOX = 0, 1
 
 ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "ESTATE", -5: "ENUMERIC"}

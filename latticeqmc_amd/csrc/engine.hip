@@ -42,6 +42,7 @@ void launch_flip(int fp32, int delayed, const FlipArgs& a, int nchains, int thre
                  hipStream_t s);
 size_t flip_delayed_shmem(int nmax, int k, int threads, int fp32);
 size_t flip_delayed_workspace(int n, int k, int fp32, int groups);
+size_t flip_rank1_split_shmem(int nmax, int threads, int groups);
 int flip_grid(int nchains, int groups);
 int flip_delayed_threads_limit(int k);
 size_t qr_blocked_shmem(int nmax);
@@ -176,7 +177,7 @@ static void carve(char* base, size_t& off, T*& p, size_t count) {
 
 static void layout_chain(const lqmc_ctx* x, Chain& c, char* base, size_t& off) {
   const size_t N = c.N, L = c.L, n2 = N * N;
-  const bool fp32 = x->opt.precision == LQMC_FP32;
+  const bool fp32 = x->opt.precision != LQMC_FP64;
   const bool dq = x->opt.mode == LQMC_MODE_DQMC;
   const size_t esz = fp32 ? 4 : 8;
   carve(base, off, c.field, N * L);
@@ -328,7 +329,7 @@ static void build_program(lqmc_ctx* x) {
   x->init_ops.clear();
   x->sweep_ops.clear();
   Builder b(x);
-  const bool fp32 = x->opt.precision == LQMC_FP32;
+  const bool fp32 = x->opt.precision != LQMC_FP64;
   const bool ref = x->opt.mode == LQMC_MODE_REFERENCE;
   const int ns = x->opt.n_stab;
   const int nc = x->nchains;
@@ -625,7 +626,7 @@ static void build_program(lqmc_ctx* x) {
 // ------------------------------------------------------------------------------------------
 // executor
 static void run_ops(lqmc_ctx* x, const std::vector<Op>& ops, int measure, hipStream_t s, bool profile) {
-  const bool fp32 = x->opt.precision == LQMC_FP32;
+  const bool fp32 = x->opt.precision != LQMC_FP64;
   for (const Op& op : ops) {
     if (op.meas_only && !measure) continue;
     int pk = -1;
@@ -656,7 +657,7 @@ static void run_ops(lqmc_ctx* x, const std::vector<Op>& ops, int measure, hipStr
         break;
       case OP_FLIP: {
         FlipArgs a{x->d_chains, x->d_state, op.t, measure, x->opt.rng, x->opt.trace, x->opt.delay,
-                   x->flip_groups, x->d_bar, x->nchains, 0};
+                   x->flip_groups, x->d_bar, x->nchains, x->opt.precision == LQMC_BF16X3 ? 1 : 0};
         launch_flip(fp32, x->opt.delay > 1, a, x->nchains, x->flip_threads, x->flip_shmem, s);
         break;
       }
@@ -684,7 +685,10 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
     return fail(LQMC_EINVAL, "unknown mode");
   if (o.mode == LQMC_MODE_DET && (o.precision != LQMC_FP64 || o.delay != 1))
     return fail(LQMC_EINVAL, "determinant mode runs the FP64 rank-1 sweep only");
-  if (o.precision != LQMC_FP64 && o.precision != LQMC_FP32) return fail(LQMC_EINVAL, "unknown precision");
+  if (o.precision != LQMC_FP64 && o.precision != LQMC_FP32 && o.precision != LQMC_BF16X3)
+    return fail(LQMC_EINVAL, "unknown precision");
+  if (o.precision == LQMC_BF16X3 && (o.mode != LQMC_MODE_DQMC || o.delay < 4))
+    return fail(LQMC_EINVAL, "the split-BF16 form needs the Mode D delayed flip (delay >= 4)");
   if (o.rng != LQMC_RNG_REPLAY && o.rng != LQMC_RNG_PHILOX) return fail(LQMC_EINVAL, "unknown rng");
   if (o.delay < 1 || o.delay > 64) return fail(LQMC_EINVAL, "delay must be in 1..64");
   if (o.delay != 1 && o.delay != 4 && o.delay != 8 && o.delay != 16 && o.delay != 32)
@@ -726,7 +730,7 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
   // delayed form's rank-k operands live in a per-chain global workspace), which
   // bounds the supported delay for a given thread count.
   const int maxsh = 65536;
-  const bool fp32 = o.precision == LQMC_FP32;
+  const bool fp32 = o.precision != LQMC_FP64;
   const int tmax = o.delay > 1 ? flip_delayed_threads_limit(o.delay) : 1024;
   if (o.flip_threads > 0) x->flip_threads = o.flip_threads;
   else x->flip_threads = x->nmax >= 128 ? tmax : 256;
@@ -745,9 +749,10 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
     delete x;
     return fail(LQMC_EINVAL, buf);
   }
-  // Workgroups per chain of the delayed flip (blocks of 16 or 32 sites, whole 16-row strips):
-  // all of them must be resident at once, since they wait for each other between blocks, so
-  // the grid stays within one workgroup per compute unit.
+  // Workgroups per chain of the flip (Mode D: the delayed form with blocks of 16 or 32 sites,
+  // whole 16-row strips, or the rank-1 form, rows split evenly): all of them must be resident
+  // at once, since they wait for each other, so the grid stays within one workgroup per
+  // compute unit.
   {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
@@ -757,7 +762,7 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
     const int cus = prop.multiProcessorCount;
     x->cus = cus;
     const int padded = 8 * ((n_chains + 7) / 8);
-    const bool splittable = o.mode == LQMC_MODE_DQMC && (o.delay == 16 || o.delay == 32);
+    const bool splittable = o.mode == LQMC_MODE_DQMC && (o.delay == 1 || o.delay == 16 || o.delay == 32);
     const int waves = x->flip_threads / 64;
     if (o.flip_groups < 0 || o.flip_groups > 64) {
       delete x;
@@ -765,7 +770,7 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
     }
     if (o.flip_groups > 1 && !splittable) {
       delete x;
-      return fail(LQMC_EINVAL, "flip_groups > 1 needs the Mode D delayed flip with delay 16 or 32");
+      return fail(LQMC_EINVAL, "flip_groups > 1 needs the Mode D flip with delay 1, 16 or 32");
     }
     if (o.flip_groups > 1 && flip_grid(n_chains, o.flip_groups) > cus) {
       char buf[160];
@@ -775,11 +780,22 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
       return fail(LQMC_EINVAL, buf);
     }
     if (o.flip_groups > 0) x->flip_groups = o.flip_groups;
-    else if (splittable) {
+    else if (splittable && o.delay > 1) {
       // at least one 16-row strip of either spin per wavefront
       const int strips = 2 * ((x->nmax + 15) / 16);
       x->flip_groups = std::max(1, std::min(cus / padded, strips / waves));
+    } else if (splittable) {
+      // rank-1: at least 32 rows of each spin per workgroup
+      x->flip_groups = std::max(1, std::min(cus / padded, x->nmax / 32));
     } else x->flip_groups = 1;
+    if (o.delay == 1 && x->flip_groups > 1) {
+      if (o.flip_threads <= 0) x->flip_threads = 512;
+      x->flip_shmem = flip_rank1_split_shmem(x->nmax, x->flip_threads, x->flip_groups);
+      if ((int)x->flip_shmem > maxsh) {
+        delete x;
+        return fail(LQMC_EINVAL, "split rank-1 flip needs more than 64 KB of LDS");
+      }
+    }
   }
   if (o.mode == LQMC_MODE_DET) {
     x->det_shmem = det_sweep_shmem(x->nmax, x->lmax);
@@ -875,7 +891,7 @@ int lqmc_set_chain(lqmc_ctx* x, int ci, const double* expk, const double* expk_i
   HIPCHK(hipMemcpy(c.hop, hop.data(), n2 * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c.eye, eye.data(), n2 * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c.ones, ones.data(), N * 8, hipMemcpyHostToDevice));
-  if (x->opt.precision == LQMC_FP32) {
+  if (x->opt.precision != LQMC_FP64) {
     std::vector<float> a(n2), b(n2);
     for (size_t e = 0; e < n2; ++e) { a[e] = (float)expk[e]; b[e] = (float)expk_inv[e]; }
     HIPCHK(hipMemcpy(c.expk32, a.data(), n2 * 4, hipMemcpyHostToDevice));
@@ -1004,7 +1020,7 @@ static int green_src(lqmc_ctx* x, int ci, int which, std::vector<double>& out) {
   out.resize(2 * n2);
   HIPCHK(hipStreamSynchronize(x->stream));
   if (which == 0) {
-    if (x->opt.precision == LQMC_FP32) {
+    if (x->opt.precision != LQMC_FP64) {
       std::vector<float> f(2 * n2);
       HIPCHK(hipMemcpy(f.data(), c.G, 4 * 2 * n2, hipMemcpyDeviceToHost));
       for (size_t e = 0; e < 2 * n2; ++e) out[e] = f[e];
@@ -1046,7 +1062,7 @@ int lqmc_get_green_dev(lqmc_ctx* x, int ci, double* dev_out, int which, void* st
     HIPCHK(hipEventRecord(x->aux_ev, x->stream));
     HIPCHK(hipStreamWaitEvent(s, x->aux_ev, 0));
   }
-  const bool fp32 = which == 0 && x->opt.precision == LQMC_FP32;
+  const bool fp32 = which == 0 && x->opt.precision != LQMC_FP64;
   const void* src = which == 0 ? c.G : (which == 1 ? (const void*)c.Gacc : (const void*)c.Glast);
   launch_green_out(dev_out, src, fp32 ? 1 : 0, n, x->d_state + ci, which == 1 ? 1 : 0, s);
   HIPCHK(hipGetLastError());

@@ -229,17 +229,41 @@ __global__ __launch_bounds__(1024) void k_flip_rank1(FlipArgs a) {
     }
     __syncthreads();
     {
+      // RB rows per wavefront at a time, every load of a batch issued before its arithmetic
+      constexpr int RB = 4;
       const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-      for (int row = threadIdx.x >> 6; row < 2 * N; row += nw) {
-        const int sp = row >= N ? 1 : 0;
-        const int j = row - sp * N;
-        const double xj = xv[sp * N + j];
-        const double cj = coef[sp];
-        T* Gr = G + sp * n2 + (size_t)j * N;
-        const double* y = yv + sp * N;
-        for (int k = lane; k < N; k += 64) {
-          const double p = ref ? xj * y[k] : cj * (xj * y[k]);
-          Gr[k] = (T)((double)Gr[k] - p);
+      for (int row0 = (threadIdx.x >> 6) * RB; row0 < 2 * N; row0 += nw * RB) {
+        for (int k0 = 0; k0 < N; k0 += 256) {
+          double gv[RB][4];
+#pragma unroll
+          for (int b = 0; b < RB; ++b) {
+            const int row = row0 + b;
+            const bool rok = row < 2 * N;
+            const LQMC_GLOBAL T* Gr = gptr(G + (rok ? (size_t)row * N : 0));
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int k = k0 + 64 * q + lane;
+              gv[b][q] = (double)ldg0(Gr, (size_t)k, rok && k < N);
+            }
+          }
+#pragma unroll
+          for (int b = 0; b < RB; ++b) {
+            const int row = row0 + b;
+            if (row >= 2 * N) continue;
+            const int sp = row >= N ? 1 : 0;
+            const int j = row - sp * N;
+            const double xj = xv[sp * N + j];
+            const double cj = coef[sp];
+            T* Gr = G + sp * n2 + (size_t)j * N;
+            const double* y = yv + sp * N;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int k = k0 + 64 * q + lane;
+              if (k >= N) continue;
+              const double p = ref ? xj * y[k] : cj * (xj * y[k]);
+              Gr[k] = (T)(gv[b][q] - p);
+            }
+          }
         }
       }
     }
@@ -268,6 +292,37 @@ template <> struct MfmaOp<float> {
   }
   __device__ static inline int row(int lane, int r) { return ((lane >> 4) << 2) + r; }
 };
+
+// ---- split-BF16 products -----------------------------------------------------------------------
+// x = x_hi + x_lo with x_hi = bf16(x) (round to nearest even) and x_lo = bf16(x - x_hi); a
+// product a b is taken as a_hi b_hi + a_hi b_lo + a_lo b_hi (the a_lo b_lo term, ~2^-16 of it,
+// dropped), each an exact BF16 product summed in FP32 on the matrix cores.
+typedef short v4s __attribute__((ext_vector_type(4)));
+
+__device__ inline unsigned short bf16_rne(float x) {
+  const unsigned u = __float_as_uint(x);
+  return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ inline float bf16_to_f32(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+// The four FP32 values of one lane (k = 4 (lane / 16) + j of a 16x16x16 operand) as hi and lo halves.
+__device__ inline void bf16_split4(const float (&x)[4], v4s& hi, v4s& lo) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const unsigned short h = bf16_rne(x[j]);
+    hi[j] = (short)h;
+    lo[j] = (short)bf16_rne(x[j] - bf16_to_f32(h));
+  }
+}
+
+// acc += A B over k = 16 as three BF16 products (16x16x16, FP32 accumulation); A lane l holds
+// A[l % 16][4 (l / 16) + j], B lane l holds B[4 (l / 16) + j][l % 16], j = 0..3.
+__device__ inline v4f mma_bf16x3(const v4s& ah, const v4s& al, const v4s& bh, const v4s& bl, v4f acc) {
+  acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
+  return acc;
+}
 
 // Orders LDS traffic between the lanes of one wavefront (the serial phase runs in wave 0).
 __device__ inline void wave_sync() {
@@ -356,7 +411,7 @@ __device__ void form_z(const T* G, size_t n2, int N, int Np, int b0, int kb, int
 // formed from G0[strip rows, B], or from the 16 x 16 copy pin of it when given (a strip
 // whose column groups are shared by several wavefronts: one of them may already have
 // updated the tile holding columns B).
-template <typename T, int K, int RU>
+template <typename T, int K, int RU, bool BF3 = false>
 __device__ void update_strip(T* G, size_t n2, int N, int Np, int b0, int kb, int m, const double* Mx,
                              const T* zws, int rs, int cg = 0, int ng = 1, const double* pin = nullptr) {
   using acc_t = typename MfmaOp<T>::acc_t;
@@ -407,14 +462,33 @@ __device__ void update_strip(T* G, size_t n2, int N, int Np, int b0, int kb, int
           zv[u][jt][v] = ldg0(Zg, ((size_t)((sp * nct + ct) * JT + jt) * 4 + v) * 64 + lane,
                               ct < nct && jt < jtu);
     }
-#pragma unroll
-    for (int u = 0; u < RU; ++u)
+    if constexpr (BF3) {
+      // FP32 operands: lane l's P^T accumulator elements v = 0..3 are P[l % 16][4 (l / 16) + v] and
+      // its Z elements are Z[4 (l / 16) + v][l % 16] -- the k = 16 operand layout of the BF16 product
 #pragma unroll
       for (int jt = 0; jt < JT; ++jt)
         if (jt < jtu) {
+          const float pv[4] = {(float)pt[jt][0], (float)pt[jt][1], (float)pt[jt][2], (float)pt[jt][3]};
+          v4s ph, pl;
+          bf16_split4(pv, ph, pl);
 #pragma unroll
-          for (int v = 0; v < 4; ++v) acc[u] = MfmaOp<T>::mma(pt[jt][v], zv[u][jt][v], acc[u]);
+          for (int u = 0; u < RU; ++u) {
+            const float zz[4] = {(float)zv[u][jt][0], (float)zv[u][jt][1], (float)zv[u][jt][2], (float)zv[u][jt][3]};
+            v4s zh, zl;
+            bf16_split4(zz, zh, zl);
+            acc[u] = mma_bf16x3(ph, pl, zh, zl, acc[u]);
+          }
         }
+    } else {
+#pragma unroll
+      for (int u = 0; u < RU; ++u)
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt)
+          if (jt < jtu) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u] = MfmaOp<T>::mma(pt[jt][v], zv[u][jt][v], acc[u]);
+          }
+    }
 #pragma unroll
     for (int u = 0; u < RU; ++u) {
       const int ct = c0 + u;
@@ -467,6 +541,29 @@ __device__ inline void chain_barrier(unsigned* bar, unsigned target, ChainState*
   __syncthreads();
 }
 
+// The reads-before-writes half of a step: every workgroup of the chain has finished reading the
+// state it decides and updates from (its loads have returned: their values are in LDS or were
+// consumed) before any of them writes.  No memory is published, so no fences; the counter is
+// the barrier counter, whose targets it shares.
+__device__ inline void chain_sync_reads(unsigned* bar, unsigned target, ChainState* st, bool& broken) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(bar, 1u);
+    if (!broken) {
+      unsigned spins = 0;
+      while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins >= FLIP_BARRIER_SPINS) {
+          broken = true;
+          atomicOr(&st->flags, 16u);
+          break;
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
 __device__ inline void chain_exit(unsigned* bar, int W) {
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -497,7 +594,7 @@ __device__ inline void flip_block_map(const FlipArgs& a, int& c, int& w) {
   w = slot % W;
 }
 
-template <typename T, int K>
+template <typename T, int K, bool BF3 = false>
 __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(FlipArgs a) {
   static_assert(K >= 4 && K <= 32 && K % 4 == 0, "block size");
   int c, w;
@@ -560,7 +657,7 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
         if ((rs % tw) / nw != w) continue;
         for (int g = 0; g < ngr; ++g, ++item)
           if (wave == w0 + item % nwv)
-            update_strip<T, K, RUd>(G, n2, N, Np, pb0, pkb, pm, Mxp, wsT, rs, g, ngr, Pin + sp * 256);
+            update_strip<T, K, RUd, BF3>(G, n2, N, Np, pb0, pkb, pm, Mxp, wsT, rs, g, ngr, Pin + sp * 256);
       }
     } else {
       int i = 0;
@@ -569,7 +666,7 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
           const int rs = sp * nct + st, own = rs % tw;
           if (own / nw != w) continue;
           const int dw = serial_busy ? 1 + i % (nw - 1) : own % nw;
-          if (wave == dw) update_strip<T, K, RUd>(G, n2, N, Np, pb0, pkb, pm, Mxp, wsT, rs);
+          if (wave == dw) update_strip<T, K, RUd, BF3>(G, n2, N, Np, pb0, pkb, pm, Mxp, wsT, rs);
         }
     }
   };
@@ -781,7 +878,7 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
       const int s0 = defer ? b0 >> 4 : nct, s1 = defer ? (b0 + kb - 1) >> 4 : -1;
       for (int rs = gw; rs < 2 * nct; rs += tw) {
         const int st = rs % nct;
-        if (st < s0 || st > s1) update_strip<T, K, (K >= 16 ? 4 : 2)>(G, n2, N, Np, b0, kb, m, Xsb, wsT, rs);
+        if (st < s0 || st > s1) update_strip<T, K, (K >= 16 ? 4 : 2), BF3>(G, n2, N, Np, b0, kb, m, Xsb, wsT, rs);
       }
     }
     if (K >= 16 && W > 1) {
@@ -1018,6 +1115,153 @@ __global__ __launch_bounds__(256) void k_det_sweep(DetArgs a) {
   if (threadIdx.x == 0) a.st[c].sweep += 1ull;
 }
 
+// ---- rank-1 form on several workgroups per chain -----------------------------------------------
+// The plain sweep with the compute-unit budget of the delayed form: W workgroups per chain,
+// workgroup w owning rows [w N / W, (w + 1) N / W) of both spins' G.  Per site every workgroup
+// reads G_ii of both spins (current: a chain barrier ends every accepted flip), and forms the same
+// ratio and decision from the same uniform.  On acceptance each workgroup takes row i of both
+// spins into LDS and the column entries G[r, i] of its own rows, waits until every workgroup has
+// done its reads of this state (the diagonals of the rejected sites before i included: a
+// workgroup ahead must not overwrite them), and applies the update of k_flip_rank1,
+// G -= c x (e_i - G[i, :])^T (the same operations in the same order, so W = 1 and W > 1 agree
+// bit for bit), to its rows except row i, which every workgroup reads during the step: its
+// owner keeps the updated row in LDS and writes it after the barrier that ends the step.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_flip_rank1_split(FlipArgs a) {
+  int c, w;
+  flip_block_map(a, c, w);
+  const int W = a.wpc;
+  if (c >= a.nchains) return;
+  const ChainDev ch = a.chains[c];
+  if (a.t >= ch.L) return;
+  const int N = ch.N;
+  const int l = ch.L - 1 - a.t;
+  const size_t n2 = (size_t)N * N;
+  T* __restrict__ G = static_cast<T*>(ch.G);
+  int8_t* fld = ch.field + (size_t)l * N;
+  const int r0 = (int)((long)N * w / W), r1 = (int)((long)N * (w + 1) / W), nr = r1 - r0;
+  LQMC_DYNAMIC_LDS(double, sm);
+  double* grow = sm;               // [2][N] row i of both spins before the update
+  double* prow = grow + 2 * N;     // [2][N] the owner's updated row of the pending site
+  double* xv = prow + 2 * N;       // [2][nr] G[r, i] of my rows
+  double* red = xv + 2 * nr;       // blockDim (epilogue)
+  double* coef = red + blockDim.x;  // [2] update coefficients of both spins
+  int8_t* fl = reinterpret_cast<int8_t*>(coef + 2);  // [N] the slice's field
+  unsigned* bar = a.bar + 2 * c;
+  unsigned nbar = 0;
+  bool broken = false, underrun = false, bad = false;
+  const unsigned long long sweep = a.st[c].sweep;
+  unsigned long long nacc = 0;
+  int pend = -1;  // site whose updated row waits in prow (uniform)
+  for (int i = threadIdx.x; i < N; i += blockDim.x) fl[i] = fld[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  for (int i = 0; i < N; ++i) {
+    const int s = fl[i];
+    const double gu = (double)G[(size_t)i * N + i];
+    const double gd = (double)G[n2 + (size_t)i * N + i];
+    const double e2p = ch.e2p, e2m = ch.e2m;
+    const double ep = s > 0 ? e2p : e2m;
+    const double em = s > 0 ? e2m : e2p;
+    const double d_up = 1.0 + (1.0 - gu) * (ep - 1.0);
+    const double d_dn = 1.0 + (1.0 - gd) * (em - 1.0);
+    const double ratio = d_up * d_dn;
+    const double u = draw_uniform(ch, &a.st[c], c, a.rng, sweep, a.t, i, underrun);
+    const bool acc = u <= ratio;
+    if (!isfinite(ratio)) bad = true;
+    if (a.trace && threadIdx.x == 0 && w == 0) {
+      ch.tr_ratio[(size_t)a.t * N + i] = ratio;
+      ch.tr_acc[(size_t)a.t * N + i] = acc ? 1 : 0;
+    }
+    if (!acc) continue;
+    // the pending row of the previous accepted site: nobody reads it any more
+    if (pend >= r0 && pend < r1)
+      for (int j = threadIdx.x; j < 2 * N; j += blockDim.x) {
+        const int sp = j / N, jj = j % N;
+        G[sp * n2 + (size_t)pend * N + jj] = (T)prow[j];
+      }
+    __syncthreads();
+    for (int j = threadIdx.x; j < 2 * N; j += blockDim.x) {
+      const int sp = j / N, jj = j % N;
+      grow[j] = (double)G[sp * n2 + (size_t)i * N + jj];
+    }
+    for (int j = threadIdx.x; j < 2 * nr; j += blockDim.x) {
+      const int sp = j / nr, r = r0 + j % nr;
+      xv[j] = (double)G[sp * n2 + (size_t)r * N + i];
+    }
+    if (threadIdx.x < 2) {
+      const double al = threadIdx.x == 0 ? ep - 1.0 : em - 1.0;
+      const double de = threadIdx.x == 0 ? d_up : d_dn;
+      coef[threadIdx.x] = al / de;
+    }
+    chain_sync_reads(bar, ++nbar * (unsigned)W, &a.st[c], broken);
+    // G[r, :] -= c (x_r (e_i - G[i, :])): each wavefront takes RB rows at a time, lanes over
+    // 256-column chunks, with every load of a batch issued before its arithmetic (the rows
+    // stream from memory; one load at a time would leave the update latency-bound)
+    {
+      constexpr int RB = 4;
+      for (int row0 = (threadIdx.x >> 6) * RB; row0 < 2 * nr; row0 += nw * RB) {
+        for (int k0 = 0; k0 < N; k0 += 256) {
+          double gv[RB][4];
+#pragma unroll
+          for (int b = 0; b < RB; ++b) {
+            const int row = row0 + b;
+            const bool rok = row < 2 * nr;
+            const int sp = rok ? row / nr : 0, r = r0 + (rok ? row % nr : 0);
+            const LQMC_GLOBAL T* Gr = gptr(G + sp * n2 + (size_t)r * N);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int k = k0 + 64 * q + lane;
+              gv[b][q] = (double)ldg0(Gr, (size_t)k, rok && r != i && k < N);
+            }
+          }
+#pragma unroll
+          for (int b = 0; b < RB; ++b) {
+            const int row = row0 + b;
+            if (row >= 2 * nr) continue;
+            const int sp = row / nr, r = r0 + row % nr;
+            const double xj = xv[row];
+            const double cj = coef[sp];
+            const double* gi = grow + sp * N;
+            T* Gr = G + sp * n2 + (size_t)r * N;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int k = k0 + 64 * q + lane;
+              if (k >= N) continue;
+              double y = -gi[k];
+              if (k == i) y = y + 1.0;
+              if (r == i) prow[sp * N + k] = gi[k] - cj * (xj * y);
+              else Gr[k] = (T)(gv[b][q] - cj * (xj * y));
+            }
+          }
+        }
+      }
+    }
+    pend = i;
+    if (threadIdx.x == 0) fl[i] = (int8_t)(-s);
+    ++nacc;
+    chain_barrier(bar, ++nbar * (unsigned)W, &a.st[c], broken);
+  }
+  if (pend >= r0 && pend < r1) {
+    for (int j = threadIdx.x; j < 2 * N; j += blockDim.x) {
+      const int sp = j / N, jj = j % N;
+      G[sp * n2 + (size_t)pend * N + jj] = (T)prow[j];
+    }
+  }
+  // every row final (and the sweep counter read by every workgroup) before workgroup 0 measures
+  if (l == 0) chain_barrier(bar, ++nbar * (unsigned)W, &a.st[c], broken);
+  if (w == 0) {
+    for (int i = threadIdx.x; i < N; i += blockDim.x) fld[i] = fl[i];
+    slice_epilogue<T>(a, ch, c, l, G, nacc, underrun, bad, red);
+  }
+  chain_exit(bar, W);
+}
+
+size_t flip_rank1_split_shmem(int nmax, int threads, int groups) {
+  const size_t nr = ((size_t)nmax + groups - 1) / groups;
+  return sizeof(double) * (4 * (size_t)nmax + 2 * nr + threads + 2) + (size_t)nmax;
+}
+
 size_t det_sweep_shmem(int nmax, int lmax) {
   return sizeof(double) * (5 * (size_t)nmax * nmax + 256 + 2) + sizeof(int) * 2 * DET_NMAX + (size_t)nmax * lmax;
 }
@@ -1053,10 +1297,11 @@ void launch_flip(int fp32, int delayed, const FlipArgs& a, int nchains, int thre
                  hipStream_t s) {
   if (delayed) {
     const int grid = flip_grid(nchains, a.wpc);
-#define LQMC_DELAYED(KK)                                                                               \
-  case KK:                                                                                             \
-    if (fp32) hipLaunchKernelGGL((k_flip_delayed<float, KK>), dim3(grid), dim3(threads), shmem, s, a); \
-    else hipLaunchKernelGGL((k_flip_delayed<double, KK>), dim3(grid), dim3(threads), shmem, s, a);    \
+#define LQMC_DELAYED(KK)                                                                                    \
+  case KK:                                                                                                  \
+    if (fp32 && a.bf3) hipLaunchKernelGGL((k_flip_delayed<float, KK, true>), dim3(grid), dim3(threads), shmem, s, a); \
+    else if (fp32) hipLaunchKernelGGL((k_flip_delayed<float, KK>), dim3(grid), dim3(threads), shmem, s, a); \
+    else hipLaunchKernelGGL((k_flip_delayed<double, KK>), dim3(grid), dim3(threads), shmem, s, a);         \
     break;
     switch (a.delay) {
       LQMC_DELAYED(4)
@@ -1066,6 +1311,10 @@ void launch_flip(int fp32, int delayed, const FlipArgs& a, int nchains, int thre
       default: break;
     }
 #undef LQMC_DELAYED
+  } else if (a.wpc > 1) {
+    const int grid = flip_grid(nchains, a.wpc);
+    if (fp32) hipLaunchKernelGGL(k_flip_rank1_split<float>, dim3(grid), dim3(threads), shmem, s, a);
+    else hipLaunchKernelGGL(k_flip_rank1_split<double>, dim3(grid), dim3(threads), shmem, s, a);
   } else {
     if (fp32) hipLaunchKernelGGL(k_flip_rank1<float>, dim3(nchains), dim3(threads), shmem, s, a);
     else hipLaunchKernelGGL(k_flip_rank1<double>, dim3(nchains), dim3(threads), shmem, s, a);

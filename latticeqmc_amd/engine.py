@@ -14,7 +14,7 @@ from scipy.linalg import expm
 from . import _native as nat
 
 _MODES = {"dqmc": nat.MODE_DQMC, "reference": nat.MODE_REFERENCE, "det": nat.MODE_DET}
-_PREC = {"fp64": nat.FP64, "fp32": nat.FP32}
+_PREC = {"fp64": nat.FP64, "fp32": nat.FP32, "bf16x3": nat.BF16X3}
 _RNG = {"replay": nat.RNG_REPLAY, "philox": nat.RNG_PHILOX}
 
 
@@ -60,7 +60,8 @@ class SweepEngine:
               determinant sweep, lqmc.py:236-299, N <= 32) or "reference" (literal
               lqmc/lqmc.py:301-347 math, FP64 rank-1, unstabilised)
     precision "fp64" or "fp32" (storage of G, wraps and flip updates; the
-              stabilisation always runs in FP64)
+              stabilisation always runs in FP64), or "bf16x3": the FP32 form with the
+              delayed rank-k products as three split-BF16 matrix-core products
     delay     1: rank-1 per accepted flip; k > 1: submatrix blocks of k sites
     n_stab    slices between FP64 recomputes
     rng       "replay" (uniforms supplied per sweep) or "philox" (on device)

@@ -13,6 +13,9 @@ Extra keyword-only options select the GPU form:
                     reference's own fast-sweep math (lqmc.py:301-347), which is
                     only consistent at U = 0
   precision, delay, n_stab, rng, seed, device, graph -- see ``SweepEngine``.
+  delay=None (default) picks the delayed form where it pays: 16-site blocks for
+  N >= 64, 8 for N >= 16, the rank-1 update below; graph=None replays whole
+  sweeps from a captured hipGraph except in determinant mode.
   rng="numpy" (default) draws the uniforms from numpy's global RNG in the
   reference's order, one per attempted flip, so a seeded run consumes the same
   stream as the reference; rng="philox" draws them on the device.
@@ -36,8 +39,8 @@ log = logging.getLogger("latticeqmc_amd")
 
 class LatticeQMC:
     def __init__(self, model, beta, time_steps, warmup=300, sweeps=2000, det_mode=False, log_lvl=None,
-                 *, mode="dqmc", precision="fp64", delay=1, n_stab=10, rng="numpy", seed=None,
-                 device=0, graph=False, console=True, time_displaced=False):
+                 *, mode="dqmc", precision="fp64", delay=None, n_stab=10, rng="numpy", seed=None,
+                 device=0, graph=None, console=True, time_displaced=False):
         if log_lvl is not None:
             log.setLevel(log_lvl)
         self.model = model
@@ -57,9 +60,13 @@ class LatticeQMC:
         self.lamb = 0.0
         self.exp_k = None
         self.exp_k_inv = None
+        if delay is None:
+            delay = 1 if mode != "dqmc" else (16 if self.n_sites >= 64 else 8 if self.n_sites >= 16 else 1)
         self.mode, self.precision, self.delay, self.n_stab = mode, precision, delay, n_stab
         if det_mode:  # the reference's determinant sweep (lqmc.py:236-299) on the device
             self.mode, self.precision, self.delay = "det", "fp64", 1
+        if graph is None:
+            graph = self.mode != "det"
         self.rng = rng
         self.seed = seed if seed is not None else int(np.random.randint(0, 2**31 - 1)) if rng == "philox" else 0
         self.device = device

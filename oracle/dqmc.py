@@ -80,8 +80,12 @@ def identity_udv(n):
 class DQMCOracle:
     """One Markov chain of the Mode D sweep in FP64."""
 
-    def __init__(self, k, beta, L, u, field, n_stab=10, delay=1, expk=None, expk_inv=None):
+    def __init__(self, k, beta, L, u, field, n_stab=10, delay=1, expk=None, expk_inv=None, dtype=np.float64):
+        """dtype=np.float32 keeps G, the wraps and the flip updates in FP32 (E_K rounded to FP32),
+        with the ratios, coefficients and every recompute in FP64 -- the FP32 form of the engine;
+        ``errors`` then records max|G_FP32 - G_FP64| at each recompute."""
         from .model import params
+        self.dtype = np.dtype(dtype)
         self.k = np.asarray(k, dtype=np.float64)
         self.n = self.k.shape[0]
         self.L = int(L)
@@ -98,6 +102,10 @@ class DQMCOracle:
         self.errors = []
         self.g = [None, None]
         self.init_g()
+        if self.dtype != np.float64:
+            self.g = [g.astype(self.dtype) for g in self.g]
+            self._ek_f = self.expk.astype(self.dtype)
+            self._eki_f = self.expk_inv.astype(self.dtype)
 
     # -- slice matrices ------------------------------------------------------------------
     def v(self, l, sigma):
@@ -154,10 +162,10 @@ class DQMCOracle:
         for s_i in range(2):
             g = self.g[s_i]
             coef = alphas[s_i] / dets[s_i]
-            x = g[:, i].copy()
-            y = -g[i, :].copy()
+            x = g[:, i].astype(np.float64)
+            y = -g[i, :].astype(np.float64)
             y[i] += 1.0
-            self.g[s_i] = g - coef * np.outer(x, y)
+            self.g[s_i] = (g - coef * np.outer(x, y)).astype(self.dtype)
 
     def slice_rank1(self, l, uni, rec):
         for i in range(self.n):
@@ -172,7 +180,7 @@ class DQMCOracle:
         k = self.delay
         for b0 in range(0, self.n, k):
             blk = np.arange(b0, min(b0 + k, self.n))
-            small = [self.g[0][np.ix_(blk, blk)].copy(), self.g[1][np.ix_(blk, blk)].copy()]
+            small = [self.g[0][np.ix_(blk, blk)].astype(np.float64), self.g[1][np.ix_(blk, blk)].astype(np.float64)]
             acc_idx, acc_alpha = [], [[], []]
             for jj, i in enumerate(blk):
                 s = self.field[i, l]
@@ -197,18 +205,28 @@ class DQMCOracle:
                 a = np.asarray(acc_idx)
                 for s_i in range(2):
                     g = self.g[s_i]
-                    gam = np.diag(1.0 / np.asarray(acc_alpha[s_i])) - g[np.ix_(a, a)]
+                    gam = np.diag(1.0 / np.asarray(acc_alpha[s_i])) - g[np.ix_(a, a)].astype(np.float64)
                     gam[np.diag_indices(len(a))] += 1.0
-                    p = g[:, a] @ np.linalg.inv(gam)
-                    z = -g[a, :]
-                    z[np.arange(len(a)), a] += 1.0
-                    self.g[s_i] = g - p @ z
+                    if self.dtype == np.float64:
+                        p = g[:, a] @ np.linalg.inv(gam)
+                        z = -g[a, :]
+                        z[np.arange(len(a)), a] += 1.0
+                        self.g[s_i] = g - p @ z
+                    else:  # FP64 coefficients, FP32 operands and accumulation of the rank-k product
+                        p = (g[:, a].astype(np.float64) @ np.linalg.inv(gam)).astype(self.dtype)
+                        z = -g[a, :]
+                        z[np.arange(len(a)), a] += self.dtype.type(1.0)
+                        self.g[s_i] = g - p @ z
 
     def wrap(self, l):
         for s_i, sigma in enumerate((+1, -1)):
             v = self.v(l, sigma)
-            gs = self.g[s_i] * (v[None, :] / v[:, None])      # V^-1 G V
-            self.g[s_i] = self.expk_inv @ gs @ self.expk
+            if self.dtype == np.float64:
+                gs = self.g[s_i] * (v[None, :] / v[:, None])      # V^-1 G V
+                self.g[s_i] = self.expk_inv @ gs @ self.expk
+            else:
+                gs = self.g[s_i] * (v[None, :] / v[:, None]).astype(self.dtype)
+                self.g[s_i] = self._eki_f @ gs @ self._ek_f
 
     # -- sweep -------------------------------------------------------------------------------
     def sweep(self, uniforms, max_slices=None):
@@ -232,7 +250,7 @@ class DQMCOracle:
             else:
                 self.slice_rank1(l, uniforms[t], rec)
             if l == 0:
-                g_meas = (self.g[0].copy(), self.g[1].copy())
+                g_meas = (self.g[0].astype(np.float64), self.g[1].astype(np.float64))
             self.wrap(l)
             for s_i, sigma in enumerate((+1, -1)):
                 if wacc[s_i] is None:
@@ -249,8 +267,8 @@ class DQMCOracle:
                     wacc[s_i] = None
                     left = identity_udv(n) if t == L - 1 else stacks[s_i][m_next]
                     gst = stable_g(left, right[s_i])
-                    err = max(err, float(np.max(np.abs(self.g[s_i] - gst))))
-                    self.g[s_i] = gst
+                    err = max(err, float(np.max(np.abs(self.g[s_i].astype(np.float64) - gst))))
+                    self.g[s_i] = gst.astype(self.dtype)
                     if s_i == 0:
                         gtd.append((l, []))
                     gtd[-1][1].append(stable_gtd(left, right[s_i]))

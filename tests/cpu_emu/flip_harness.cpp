@@ -2,6 +2,7 @@
 // device buffer is a separate exact-size allocation and the dynamic LDS is an
 // exact-size heap block, so any out-of-range access aborts with a report.
 // usage: flip_harness N L delay threads fp32 nchains t measure [gscale [groups]]
+// fp32: 0 FP64, 1 FP32, 2 FP32 with the rank-k products as three split-BF16 products
 // groups > 1 runs the delayed kernel with that many workgroups per chain, all blocks at once.
 #include "hip_cpu_shim.h"
 #include "../../latticeqmc_amd/csrc/flip.hip"
@@ -24,7 +25,7 @@ int main(int argc, char** argv) {
   std::uniform_real_distribution<double> U(-0.3, 0.3);
   const double gscale = argc > 9 ? atof(argv[9]) : 1.0;  // off-diagonal scale (larger -> more rejections)
   const int groups = argc > 10 ? atoi(argv[10]) : 1;
-  if (groups > 1 && K < 16) { std::fprintf(stderr, "groups > 1 needs delay 16 or 32\n"); return 2; }
+  if (groups > 1 && K > 1 && K < 16) { std::fprintf(stderr, "groups > 1 needs delay 1, 16 or 32\n"); return 2; }
   std::vector<ChainDev> chains(nch);
   std::vector<ChainState> st(nch);
   for (int c = 0; c < nch; ++c) {
@@ -61,12 +62,14 @@ int main(int argc, char** argv) {
   }
   std::memset(st.data(), 0, sizeof(ChainState) * nch);
   std::vector<unsigned> bar(2 * nch, 0u);
-  FlipArgs a{chains.data(), st.data(), t, measure, 1, 1, K, groups, bar.data(), nch, 0};
+  FlipArgs a{chains.data(), st.data(), t, measure, 1, 1, K, groups, bar.data(), nch, fp32 == 2 ? 1 : 0};
   if (K > 1 && (threads > flip_delayed_threads_limit(K) || threads < 256)) {
     std::fprintf(stderr, "threads above the delayed kernel's launch bound %d\n", flip_delayed_threads_limit(K));
     return 2;
   }
-  const size_t lds = K > 1 ? flip_delayed_shmem(N, K, threads, fp32) : sizeof(double) * (4 * (size_t)N + threads);
+  const size_t lds = K > 1 ? flip_delayed_shmem(N, K, threads, fp32)
+                   : groups > 1 ? flip_rank1_split_shmem(N, threads, groups)
+                                : sizeof(double) * (4 * (size_t)N + threads);
   std::printf("N=%d L=%d K=%d threads=%d fp32=%d chains=%d t=%d groups=%d lds=%zu\n", N, L, K, threads, fp32, nch, t,
               groups, lds);
   std::fflush(stdout);
@@ -74,12 +77,17 @@ int main(int argc, char** argv) {
 #define RUNK(KK)                                                                          \
   case KK:                                                                                \
     if (groups > 1) {                                                                     \
-      if (fp32) emu_launch_concurrent(flip_grid(nch, groups), threads, lds, [&] { k_flip_delayed<float, KK>(a); }); \
+      if (fp32 == 2) emu_launch_concurrent(flip_grid(nch, groups), threads, lds, [&] { k_flip_delayed<float, KK, true>(a); }); \
+      else if (fp32) emu_launch_concurrent(flip_grid(nch, groups), threads, lds, [&] { k_flip_delayed<float, KK>(a); }); \
       else emu_launch_concurrent(flip_grid(nch, groups), threads, lds, [&] { k_flip_delayed<double, KK>(a); });    \
-    } else if (fp32) emu_launch(nch, threads, lds, [&] { k_flip_delayed<float, KK>(a); }); \
+    } else if (fp32 == 2) emu_launch(nch, threads, lds, [&] { k_flip_delayed<float, KK, true>(a); }); \
+    else if (fp32) emu_launch(nch, threads, lds, [&] { k_flip_delayed<float, KK>(a); }); \
     else emu_launch(nch, threads, lds, [&] { k_flip_delayed<double, KK>(a); });          \
     break;
     switch (K) { RUNK(4) RUNK(8) RUNK(16) RUNK(32) default: std::printf("unsupported K\n"); return 2; }
+  } else if (groups > 1) {
+    if (fp32) emu_launch_concurrent(flip_grid(nch, groups), threads, lds, [&] { k_flip_rank1_split<float>(a); });
+    else emu_launch_concurrent(flip_grid(nch, groups), threads, lds, [&] { k_flip_rank1_split<double>(a); });
   } else {
     if (fp32) emu_launch(nch, threads, lds, [&] { k_flip_rank1<float>(a); });
     else emu_launch(nch, threads, lds, [&] { k_flip_rank1<double>(a); });
@@ -88,8 +96,9 @@ int main(int argc, char** argv) {
   for (auto& s : st) acc += s.accepted;
   for (int c = 0; c < nch; ++c)
     if (st[c].flags) { std::printf("FAIL: flags %u on chain %d\n", st[c].flags, c); return 1; }
-  // cross-check against the rank-1 kernel on a copy of the initial state (same uniforms)
-  if (K > 1 && argc > 9) {
+  // cross-check against the rank-1 kernel on a copy of the initial state (same uniforms); the
+  // rank-1 kernel split over several workgroups must agree with it bit for bit
+  if ((K > 1 || groups > 1) && argc > 9) {
     std::vector<ChainDev> ref(nch);
     std::vector<ChainState> st2(nch);
     std::memset(st2.data(), 0, sizeof(ChainState) * nch);
@@ -120,7 +129,7 @@ int main(int argc, char** argv) {
       for (size_t e = 0; e < (size_t)N * L; ++e) fdiff += chains[c].field[e] != ref[c].field[e];
     }
     std::printf("vs rank-1: max|dG| %.3e, field differences %d\n", worst, fdiff);
-    if (fdiff || worst > (fp32 ? 1e-4 : 1e-11)) { std::printf("FAIL\n"); return 1; }
+    if (fdiff || worst > (K == 1 ? 0.0 : fp32 == 2 ? 2e-3 : fp32 ? 1e-4 : 1e-11)) { std::printf("FAIL\n"); return 1; }
   }
   for (int c = 0; c < nch; ++c)
     if (st[c].flags) { std::printf("FAIL: flags %u on chain %d\n", st[c].flags, c); return 1; }

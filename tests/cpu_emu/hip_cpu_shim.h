@@ -61,7 +61,7 @@ inline uint32_t __umulhi(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)
 inline long long __double_as_longlong(double d) { long long r; std::memcpy(&r, &d, 8); return r; }
 inline double __longlong_as_double(long long v) { double r; std::memcpy(&r, &v, 8); return r; }
 
-struct EmuWave { double a[64], b[64]; std::barrier<>* bar; };
+struct EmuWave { double a[64], b[64]; short a4[64][4], b4[64][4]; std::barrier<>* bar; };
 inline thread_local EmuWave* emu_waves = nullptr;
 
 template <typename T, typename V>
@@ -84,6 +84,29 @@ inline V emu_mfma(T a, T b, V c, bool fp64) {
 }
 #define __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, x, y, z) emu_mfma<double>((a), (b), (c), true)
 #define __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, x, y, z) emu_mfma<float>((a), (b), (c), false)
+
+// 16x16x16 BF16 product with FP32 accumulation: lane l holds A[l%16][4(l/16) + j] and
+// B[4(l/16) + j][l%16] (j = 0..3, BF16 bit patterns); D as the FP32 16x16x4 product.
+template <typename S4, typename V>
+inline V emu_mfma_bf16(S4 a, S4 b, V c) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  EmuWave& s = emu_waves[w];
+  for (int j = 0; j < 4; ++j) { s.a4[lane][j] = a[j]; s.b4[lane][j] = b[j]; }
+  s.bar->arrive_and_wait();
+  auto f = [](short h) { unsigned u = (unsigned)(unsigned short)h << 16; float x; std::memcpy(&x, &u, 4); return x; };
+  V d;
+  for (int r = 0; r < 4; ++r) {
+    const int row = ((lane >> 4) << 2) + r, col = lane & 15;
+    float acc = 0.0f;
+    for (int k = 0; k < 16; ++k) acc += f(s.a4[row + 16 * (k >> 2)][k & 3]) * f(s.b4[col + 16 * (k >> 2)][k & 3]);
+    d[r] = c[r] + acc;
+  }
+  s.bar->arrive_and_wait();
+  return d;
+}
+#define __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, x, y, z) emu_mfma_bf16((a), (b), (c))
+inline unsigned __float_as_uint(float x) { unsigned u; std::memcpy(&u, &x, 4); return u; }
+inline float __uint_as_float(unsigned u) { float x; std::memcpy(&x, &u, 4); return x; }
 
 // Wavefront-level ordering: all 64 lanes of the calling wave meet.
 inline void emu_wave_sync() { emu_waves[threadIdx.x >> 6].bar->arrive_and_wait(); }

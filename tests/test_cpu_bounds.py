@@ -45,10 +45,16 @@ def harness_tsan(tmp_path_factory):
     "100 6 16 256 0 2 1 0 200 2",   # two workgroups per chain, ragged, all blocks concurrent
     "100 6 32 256 0 1 5 1 200 3",   # three workgroups per chain, k = 32, measurement slice
     "256 3 16 512 0 1 2 1 120 4",   # the throughput form: N = 256, 512 threads, four workgroups
+    "64 40 8 256 2 2 39 1 30",      # split-BF16 rank-k products (FP32 G), measurement slice
+    "64 16 16 512 2 1 15 1 40",     # split-BF16, 16-site blocks on the 512-thread bound
+    "100 4 1 256 0 3 3 1 80 3",     # rank-1 on three workgroups per chain, three chains, uneven rows
+    "256 2 1 512 0 1 1 1 100 4",    # the plain form at N = 256 on four workgroups
 ])
 def test_flip_kernel_in_bounds(harness, args):
     """In bounds under ASan; delayed runs (9 args) must match the rank-1 kernel on
-    the same state and uniforms (FP64 to 1e-11, FP32 to 1e-4; same field)."""
+    the same state and uniforms (FP64 to 1e-11, FP32 to 1e-4, split-BF16 products to 2e-3;
+    same field), and the rank-1
+    kernel split over several workgroups must match it bit for bit."""
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
     r = subprocess.run([harness, *args.split()], capture_output=True, text=True, env=env, timeout=600)
     assert r.returncode == 0 and "ok:" in r.stdout, (r.stdout + r.stderr)[-4000:]
@@ -62,6 +68,7 @@ def test_flip_kernel_in_bounds(harness, args):
     "64 16 1 256 0 1 0 1",      # rank-1
     "64 3 16 256 0 1 2 1 100 2",  # two workgroups per chain: the chain barrier orders G
     "256 2 16 512 0 1 1 1 100 2",  # the throughput form's N = 256 on 512 threads, two workgroups
+    "64 3 1 256 0 1 2 1 50 2",     # rank-1 on two workgroups: the barrier after each accepted flip
 ])
 def test_flip_kernel_race_free(harness_tsan, args):
     env = dict(os.environ, TSAN_OPTIONS="halt_on_error=0 report_signal_unsafe=0")

@@ -31,15 +31,17 @@ def _say(msg):
     print(msg, flush=True)
 
 
-@pytest.mark.parametrize("L,beta,groups,sweeps", [(16, 2.0, 0, 2), (80, 8.0, 4, 1)])
-def test_c4_chain_matches_oracle(L, beta, groups, sweeps):
-    _say(f"[c4 chain] 16x16 L={L} beta={beta} delay=16 n_stab=8 flip_groups={groups}")
+@pytest.mark.parametrize("L,beta,groups,sweeps,delay", [(16, 2.0, 0, 2, 16), (80, 8.0, 4, 1, 16), (16, 2.0, 0, 1, 1)])
+def test_c4_chain_matches_oracle(L, beta, groups, sweeps, delay):
+    """delay 16: the benchmarked form; delay 1: the plain form it is compared with, whose
+    rank-1 flip also runs on four workgroups per chain at this size."""
+    _say(f"[c4 chain] 16x16 L={L} beta={beta} delay={delay} n_stab=8 flip_groups={groups}")
     side, u, n_stab = 16, 4.0, 8
     n = side * side
     k = model.square_k(side, side)
     f = model.random_field(n, L, 0)
     o = dqmc.DQMCOracle(k, beta, L, u, f, n_stab=n_stab)
-    eng = SweepEngine([spec_from(k, beta, L, u, f)], mode="dqmc", precision="fp64", delay=16,
+    eng = SweepEngine([spec_from(k, beta, L, u, f)], mode="dqmc", precision="fp64", delay=delay,
                       n_stab=n_stab, rng="replay", trace=True, flip_groups=groups)
     g0 = eng.green(0, which=0)
     assert max(np.abs(g0[s] - o.g[s]).max() for s in range(2)) < 1e-8

@@ -99,10 +99,12 @@ def test_dqmc_fp64_matches_oracle(side, L, beta, n_stab, delay):
 
 @pytest.mark.parametrize("side,L,beta,n_stab,delay,groups", [
     (8, 40, 4.0, 10, 16, 2), (10, 20, 2.0, 10, 16, 3), (6, 20, 2.0, 10, 32, 3),
+    (8, 40, 4.0, 10, 1, 2), (10, 20, 2.0, 10, 1, 3),
 ])
 def test_dqmc_split_flip_matches_oracle(side, L, beta, n_stab, delay, groups):
     """The delayed flip with several workgroups per chain (per-block barriers, deferred
-    own-row strips) against the oracle, as the one-workgroup form."""
+    own-row strips), and the rank-1 flip split over several workgroups (a barrier per accepted
+    flip, the pending row written after it), against the oracle, as the one-workgroup forms."""
     _say(f"[dqmc split] side={side} L={L} delay={delay} groups={groups}")
     init_err, worst, st = _run_dqmc_vs_oracle(side, L, beta, 4.0, n_stab, delay, sweeps=2, flip_groups=groups)
     assert init_err < 1e-8
@@ -111,16 +113,17 @@ def test_dqmc_split_flip_matches_oracle(side, L, beta, n_stab, delay, groups):
     assert st["max_err"] < 1e-8
 
 
-def test_split_flip_batched_equals_single_group():
+@pytest.mark.parametrize("delay", [16, 1])
+def test_split_flip_batched_equals_single_group(delay):
     """Uneven chains, 2 workgroups per chain: bitwise equal to one workgroup per chain."""
-    _say("[split batched]")
+    _say(f"[split batched] delay={delay}")
     cases = [(8, 16, 2.0, 1), (6, 20, 2.0, 2), (7, 8, 1.0, 3), (10, 12, 1.5, 4)]
     specs = []
     for side, L, beta, seed in cases:
         k, f = square_case(side, L, beta, 4.0, seed)
         specs.append(spec_from(k, beta, L, 3.0 + seed, f, seed=seed))
-    one = SweepEngine(specs, mode="dqmc", delay=16, n_stab=5, rng="philox", flip_groups=1)
-    two = SweepEngine(specs, mode="dqmc", delay=16, n_stab=5, rng="philox", flip_groups=2)
+    one = SweepEngine(specs, mode="dqmc", delay=delay, n_stab=5, rng="philox", flip_groups=1)
+    two = SweepEngine(specs, mode="dqmc", delay=delay, n_stab=5, rng="philox", flip_groups=2)
     for e in (one, two):
         e.sweep(3, measure=True)
         e.sync()
@@ -243,3 +246,63 @@ def test_physics_2x2_matches_trotter_exact():
     for col, name in ((0, "docc"), (1, "ekin")):
         est, err = tools.jackknife(lambda x: x, per_chain[:, col], n_bins=32)
         assert abs(est - ex[name]) < 4 * err + 1e-4, (name, est, err, ex[name])
+
+
+@pytest.mark.parametrize("side,delay", [(4, None), (8, None), (4, 1)])
+def test_drop_in_numpy_stream_matches_oracle(side, delay):
+    """LatticeQMC(mode="dqmc", rng="numpy").run() -- by default the delayed form (8-site blocks
+    at 4x4, 16 at 8x8) replayed from captured hipGraphs -- consumes numpy's global stream in the
+    reference's order (the field at construction, then N*L uniforms per sweep, lqmc.py:309-317);
+    the oracle replays the same stream: the same final field and the measured mean G within 1e-8
+    (the return value of measure_loop, lqmc.py:356-375)."""
+    _say(f"[drop-in numpy stream] side={side} delay={delay}")
+    from latticeqmc_amd import HubbardModel, LatticeQMC
+    m = HubbardModel(u=4, t=1, mu=0)
+    m.build_square(side)
+    np.random.seed(21)
+    q = LatticeQMC(m, beta=2.0, time_steps=16, warmup=2, sweeps=3, n_stab=8, delay=delay, console=False)
+    assert q.graph and q.delay == ({4: 8, 8: 16}[side] if delay is None else delay)
+    f = q.config.config.copy()
+    state = np.random.get_state()
+    gf = q.run()
+    np.random.set_state(state)
+    o = dqmc.DQMCOracle(q.ham_kin, 2.0, 16, 4.0, f, n_stab=8, delay=q.delay)
+    n = q.n_sites
+    for _ in range(2):
+        o.sweep(np.random.rand(16 * n))
+    total = np.zeros((2, n, n))
+    for _ in range(3):
+        res = o.sweep(np.random.rand(16 * n))
+        total += np.array(res["g_meas"])
+    assert np.array_equal(q.config.config, o.field)
+    assert np.abs(gf - total / 3).max() < 1e-8
+
+
+def test_dqmc_bf16x3_tracks_fp64():
+    """The split-BF16 form (FP32 G, rank-k products as three BF16 matrix-core products) on
+    one replayed sweep: accept bits equal to the FP64 oracle's away from |ratio - u| < 1e-2,
+    every recompute within 1e-2 of the FP64 G, and a larger deviation than the FP32 form's on
+    the same trajectory (the products carry ~2^-16 relative error against FP32's 2^-24)."""
+    _say("[dqmc bf16x3]")
+    side, L, beta, n_stab = 4, 16, 2.0, 2
+    k, f = square_case(side, L, beta, 4.0, 2)
+    n = side * side
+    o = dqmc.DQMCOracle(k, beta, L, 4.0, f, n_stab=n_stab)
+    uni = np.random.RandomState(7).rand(L * n)
+    res = o.sweep(uni)
+    errs = {}
+    for prec in ("fp32", "bf16x3"):
+        eng = SweepEngine([spec_from(k, beta, L, 4.0, f)], mode="dqmc", precision=prec, delay=8,
+                          n_stab=n_stab, rng="replay", trace=True)
+        eng.set_uniforms(0, uni)
+        eng.sweep(1, measure=True)
+        eng.sync()
+        ratio, acc = eng.trace(0)
+        ok, ties = masked_equal(acc, res["accept"], res["ratio"], uni, tol=1e-2)
+        assert ok, prec
+        st = eng.stats(0)
+        assert st["flags"] == 0 and 0 < st["max_err"] < 1e-2, (prec, st["max_err"])
+        errs[prec] = st["max_err"]
+        eng.close()
+    _say(f"  max|dG| fp32 {errs['fp32']:.2e}, bf16x3 {errs['bf16x3']:.2e}")
+    assert errs["bf16x3"] > 2 * errs["fp32"]
