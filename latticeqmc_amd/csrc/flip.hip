@@ -359,8 +359,11 @@ __device__ inline void wave_sync() {
 //    issued before its first matrix-core op.
 // Phase 1 reads rows B of G0 before any strip is updated; phase 2 reads G0[strip, B] and
 // updates only its own strip.
+// spmask: bit s set = the tiles of spin s are formed (a workgroup forms only the spins of the
+// strips it updates).
 template <typename T, int K>
-__device__ void form_z(const T* G, size_t n2, int N, int Np, int b0, int kb, int m, const double* My, T* zws) {
+__device__ void form_z(const T* G, size_t n2, int N, int Np, int b0, int kb, int m, const double* My, T* zws,
+                       int spmask = 3) {
   using acc_t = typename MfmaOp<T>::acc_t;
   constexpr int JT = (K + 15) / 16;  // 16-wide tiles over the accepts
   constexpr int KS = K / 4;           // k-steps of one tile
@@ -369,10 +372,11 @@ __device__ void form_z(const T* G, size_t n2, int N, int Np, int b0, int kb, int
   const int li = lane & 15, lk = lane >> 4;
   const int nct = Np >> 4;
   const int jtu = (m + 15) >> 4;
-  const int ntile = 2 * nct * JT;
+  const int tbase = spmask == 2 ? nct * JT : 0;            // first tile formed
+  const int ntile = tbase + __builtin_popcount(spmask & 3) * nct * JT;  // one past the last
   const LQMC_GLOBAL T* Gg = gptr(G);
   LQMC_GLOBAL T* Zg = gptr(zws);
-  for (int t0 = wave; t0 < ntile; t0 += TB * nw) {
+  for (int t0 = tbase + wave; t0 < ntile; t0 += TB * nw) {
     T xv[TB][KS], yv[TB][KS];
     // every load of the group first (clamped addresses, masked values), then the products
 #pragma unroll
@@ -633,6 +637,11 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
   bool underrun = false, bad = false;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int gw = w * nw + wave, tw = W * nw;  // wavefront index within the chain, and count
+  // the spins whose strips this workgroup updates (strip rs belongs to workgroup (rs % tw) / nw);
+  // only their Z tiles are formed
+  int spmask = 0;
+  for (int rs = 0; rs < 2 * nct; ++rs)
+    if ((rs % tw) / nw == w) spmask |= 1 << (rs / nct);
   for (int i = threadIdx.x; i < N; i += blockDim.x) fl[i] = fld[i];
   __syncthreads();
   bool pending = false, broken = false;
@@ -870,7 +879,7 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
         if (r <= j) out[aidx[r] * K + j] = lower ? inv[r] : cs[r] * inv[r];
     }
     __syncthreads();
-    form_z<T, K>(G, n2, N, Np, b0, kb, m, Ysb, wsT);
+    form_z<T, K>(G, n2, N, Np, b0, kb, m, Ysb, wsT, spmask);
     __syncthreads();
     {
       // W > 1: the strips of this block's rows wait for the next barrier (see above)
