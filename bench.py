@@ -228,6 +228,8 @@ def main():
     ap.add_argument("--config", default="c4")
     ap.add_argument("--delay", type=int, default=0, help="0: autotuned/cached choice")
     ap.add_argument("--n-stab", type=int, default=0, help="0: autotuned/cached choice")
+    ap.add_argument("--flip-groups", type=int, default=-1,
+                    help="workgroups per chain in the flip; -1: autotuned/cached choice, 0: automatic")
     ap.add_argument("--autotune", action="store_true", help="tune (k, n_s) under the 1e-6 gate first")
     ap.add_argument("--tune-cache", default=None, help="autotune JSON cache path")
     ap.add_argument("--precision", default="fp64", help="fp64 | fp32 | bf16x3 (FP32 G, split-BF16 rank-k products)")
@@ -279,10 +281,16 @@ def main():
         args.delay = entry["delay"] if tuned else dflt[0]
     if not args.n_stab:
         args.n_stab = entry["n_stab"] if tuned else dflt[1]
-    log(f"[bench] chosen form: {args.precision} delay={args.delay} n_stab={args.n_stab} graph={not args.no_graph}")
+    if args.flip_groups < 0:
+        args.flip_groups = entry.get("flip_groups", 0) if tuned else 0
+    # a cache entry applies only to the form it was tuned for
+    tuned = tuned and (args.delay, args.n_stab, args.flip_groups) == (
+        entry["delay"], entry["n_stab"], entry.get("flip_groups", 0))
+    log(f"[bench] chosen form: {args.precision} delay={args.delay} n_stab={args.n_stab} "
+        f"flip_groups={args.flip_groups or 'auto'} graph={not args.no_graph} (autotuned: {tuned})")
     chosen = time_form(specs, args.steps, args.warmup, profile_kind=args.profile_kind, mode="dqmc",
                        precision=args.precision, delay=args.delay, n_stab=args.n_stab, graph=not args.no_graph,
-                       max_dg=args.max_dg)
+                       max_dg=args.max_dg, flip_groups=args.flip_groups)
     log(f"[bench] chosen: {chosen['ms_per_step']:.1f} ms/sweep, max|dG| {chosen['stats']['max_err']:.2e}")
     plain = None
     if not args.no_plain:
@@ -312,7 +320,8 @@ def main():
                    "form": f"batched, delayed rank-k (k={args.delay}), {args.precision} wraps/updates, "
                            f"FP64 QR/UDV recompute every {args.n_stab} slices, "
                            + ("hipGraph" if not args.no_graph else "no graph"),
-                   "delay": args.delay, "n_stab": args.n_stab, "autotuned": tuned},
+                   "delay": args.delay, "n_stab": args.n_stab, "flip_groups": args.flip_groups,
+                   "autotuned": tuned},
         "max_dG_vs_fp64": st["max_err"],
         "max_dG_per_recompute": chosen.get("rec"),
         "gate_1e-6": bool(st["max_err"] <= 1e-6),
