@@ -151,7 +151,7 @@ struct FlipArgs {
   int trace;
   int delay;
   int wpc;        // workgroups per chain of the delayed kernel (0 or 1: one)
-  unsigned* bar;  // [nchains][2] barrier counters (wpc > 1), zero between launches
+  unsigned* bar;  // [nchains][4] barrier counters and XCC bits (wpc > 1), zero between launches
   int nchains;
   int bf3;        // delayed FP32 form: rank-k products as three split-BF16 matrix-core products
 };

@@ -152,7 +152,7 @@ struct lqmc_ctx {
   int nmax = 0, lmax = 0;
   int flip_threads = 256;
   int flip_groups = 1;           // workgroups per chain of the delayed flip
-  unsigned* d_bar = nullptr;     // [nchains][2] their barrier counters
+  unsigned* d_bar = nullptr;     // [nchains][4] their barrier counters and XCC bits
   unsigned* d_qrpub = nullptr;   // [4 nchains] hand-over counters of the on-chip QR (two per matrix)
   int cus = 0;                   // compute units of the device
   size_t flip_shmem = 0;
@@ -825,8 +825,8 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
   if (e == hipSuccess) e = hipMemset(x->d_state, 0, sizeof(ChainState) * n_chains);
   if (e == hipSuccess) e = hipMalloc(&x->d_eye_ptrs, sizeof(double*) * n_chains);
   if (e == hipSuccess) e = hipMalloc(&x->d_eye_ns, sizeof(int) * n_chains);
-  if (e == hipSuccess) e = hipMalloc(&x->d_bar, sizeof(unsigned) * 2 * n_chains);
-  if (e == hipSuccess) e = hipMemset(x->d_bar, 0, sizeof(unsigned) * 2 * n_chains);
+  if (e == hipSuccess) e = hipMalloc(&x->d_bar, sizeof(unsigned) * 4 * n_chains);
+  if (e == hipSuccess) e = hipMemset(x->d_bar, 0, sizeof(unsigned) * 4 * n_chains);
   if (e == hipSuccess) e = hipMalloc(&x->d_qrpub, sizeof(unsigned) * 4 * n_chains);
   if (e == hipSuccess) e = hipMemset(x->d_qrpub, 0, sizeof(unsigned) * 4 * n_chains);
   if (e != hipSuccess) {

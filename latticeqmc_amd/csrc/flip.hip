@@ -516,65 +516,99 @@ __device__ void update_strip(T* G, size_t n2, int N, int Np, int b0, int kb, int
 // defers their update until after the next barrier, when no workgroup reads them any more
 // (later blocks read only their own rows B' and each strip owner its own rows).
 //
-// Barrier: counter bar[0] counts arrivals within one launch (the n-th barrier waits for n W),
-// bar[1] counts exits; the last workgroup to exit resets both, so each launch starts from
-// zero.  A wait is bounded by FLIP_BARRIER_SPINS polls; on expiry the workgroup records
-// flag 16, stops waiting for the rest of the launch and still arrives, so every wave exits.
+// Rendezvous of the W workgroups of one chain, through counters in global memory:
+// bar[0] counts arrivals within one launch (the n-th rendezvous waits for n W), bar[1] counts
+// exits (the last workgroup to exit clears bar[0..2], so each launch starts from zero), bar[2]
+// holds one bit per XCC that runs a workgroup of the chain, set at the start.  A wait is
+// bounded by FLIP_BARRIER_SPINS polls; on expiry the workgroup records flag 16, stops waiting
+// for the rest of the launch and still arrives, so every wave exits.
+//
+// A barrier publishes each workgroup's G stores to the others.  At device scope that is an L2
+// write-back before arriving and an L2 invalidate after the wait: the XCDs have separate L2s.
+// The block map gives a chain's workgroups block indices of equal residue mod 8, which is one
+// XCD under round-robin dispatch; workgroups on one XCD share its L2, so it suffices that every
+// wave's own stores have reached the L2 (vmcnt 0) before the arrival and that the acquire
+// invalidates L1.  The first barrier of a launch is always at device scope; after it every
+// workgroup reads the XCC bits and keeps the L2-local form for the rest of the launch only if
+// there is a single bit.
 constexpr unsigned FLIP_BARRIER_SPINS = 1u << 22;
 
-__device__ inline void chain_barrier(unsigned* bar, unsigned target, ChainState* st, bool& broken) {
+// XCC that runs the calling wavefront (hardware register XCC_ID, bits 3:0).
+__device__ inline unsigned xcc_id() {
+  return (unsigned)__builtin_amdgcn_s_getreg(20 | (0 << 6) | ((4 - 1) << 11)) & 15u;
+}
+
+struct ChainSync {
+  unsigned* bar;
+  ChainState* st;
+  int* flag;       // one int of LDS: the L2-local decision, broadcast to the workgroup
+  unsigned W, n;   // workgroups per chain, rendezvous so far in this launch
+  bool broken, local, decided;
+};
+
+__device__ inline ChainSync chain_sync_begin(unsigned* bar, ChainState* st, int W, int* flag) {
+  if (W > 1 && threadIdx.x == 0) atomicOr(bar + 2, 1u << xcc_id());
+  return ChainSync{bar, st, flag, (unsigned)W, 0u, false, false, false};
+}
+
+__device__ inline void chain_wait(ChainSync& cs, unsigned target) {
+  if (cs.broken) return;
+  unsigned spins = 0;
+  while (__hip_atomic_load(cs.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    __builtin_amdgcn_s_sleep(1);
+    if (++spins >= FLIP_BARRIER_SPINS) {
+      cs.broken = true;
+      atomicOr(&cs.st->flags, 16u);
+      break;
+    }
+  }
+}
+
+__device__ inline void chain_barrier(ChainSync& cs) {
+  const unsigned target = ++cs.n * cs.W;
+  if (cs.local) __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are in the shared L2
   __syncthreads();
   if (threadIdx.x == 0) {
-    // release the workgroup's G stores (ordered before this point by the barrier above) at
-    // device scope, arrive, wait, then acquire
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    atomicAdd(bar, 1u);
-    if (!broken) {
-      unsigned spins = 0;
-      while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        __builtin_amdgcn_s_sleep(1);
-        if (++spins >= FLIP_BARRIER_SPINS) {
-          broken = true;
-          atomicOr(&st->flags, 16u);
-          break;
-        }
-      }
-    }
+    // device scope: release the workgroup's G stores (ordered before this point by the
+    // barrier above), arrive, wait, then acquire
+    if (!cs.local) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    atomicAdd(cs.bar, 1u);
+    chain_wait(cs, target);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (!cs.decided) {  // every workgroup set its XCC bit before arriving here
+      const unsigned m = __hip_atomic_load(cs.bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      *cs.flag = !cs.broken && m != 0 && (m & (m - 1)) == 0;
+    }
   }
   __syncthreads();
+  if (!cs.decided) {
+    cs.local = *cs.flag != 0;
+    cs.decided = true;
+  }
 }
 
 // The reads-before-writes half of a step: every workgroup of the chain has finished reading the
 // state it decides and updates from (its loads have returned: their values are in LDS or were
 // consumed) before any of them writes.  No memory is published, so no fences; the counter is
 // the barrier counter, whose targets it shares.
-__device__ inline void chain_sync_reads(unsigned* bar, unsigned target, ChainState* st, bool& broken) {
+__device__ inline void chain_sync_reads(ChainSync& cs) {
+  const unsigned target = ++cs.n * cs.W;
   __syncthreads();
   if (threadIdx.x == 0) {
-    atomicAdd(bar, 1u);
-    if (!broken) {
-      unsigned spins = 0;
-      while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        __builtin_amdgcn_s_sleep(1);
-        if (++spins >= FLIP_BARRIER_SPINS) {
-          broken = true;
-          atomicOr(&st->flags, 16u);
-          break;
-        }
-      }
-    }
+    atomicAdd(cs.bar, 1u);
+    chain_wait(cs, target);
   }
   __syncthreads();
 }
 
-__device__ inline void chain_exit(unsigned* bar, int W) {
+__device__ inline void chain_exit(ChainSync& cs) {
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
-    if (atomicAdd(bar + 1, 1u) == (unsigned)(W - 1)) {
-      atomicExch(bar, 0u);
-      atomicExch(bar + 1, 0u);
+    if (atomicAdd(cs.bar + 1, 1u) == cs.W - 1) {
+      atomicExch(cs.bar, 0u);
+      atomicExch(cs.bar + 1, 0u);
+      atomicExch(cs.bar + 2, 0u);
     }
   }
 }
@@ -617,7 +651,7 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
   // workspace: the Z tiles of both spins in register order (2 x K16 x Np, form precision),
   // one per workgroup of the chain
   T* wsT = static_cast<T*>(ch.dws) + (size_t)w * 2 * ((K + 15) & ~15) * Np;
-  unsigned* bar = a.bar + 2 * c;
+  unsigned* bar = a.bar + 4 * c;
 
   LQMC_DYNAMIC_LDS(double, sm);
   double* red = sm;                        // blockDim (epilogue reductions)
@@ -627,7 +661,7 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
   double* ub = cf + 2 * K;                 // [K] uniforms of the block
   int* aidx = reinterpret_cast<int*>(ub + K);  // [K] block-local site of each accept
   int* sb = aidx + K;                          // [K] field entries of the block
-  int* m_sh = sb + K;                          // [4] accept count of the block
+  int* m_sh = sb + K;                          // [4] accept count of the block; [1] barrier flag
   double* Mxp = reinterpret_cast<double*>(m_sh + 4);  // [2][K][K] Mx of the previous block (W > 1)
   double* Pin = Mxp + 2 * K * K;  // [2][16][16] G0[rows, columns] of the deferred strips (K = 16)
   int8_t* fl = reinterpret_cast<int8_t*>(Pin + (K == 16 ? 512 : 0));  // [N] the slice's field
@@ -644,9 +678,9 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
     if ((rs % tw) / nw == w) spmask |= 1 << (rs / nct);
   for (int i = threadIdx.x; i < N; i += blockDim.x) fl[i] = fld[i];
   __syncthreads();
-  bool pending = false, broken = false;
+  bool pending = false;
   int pb0 = 0, pkb = 0, pm = 0;
-  unsigned nbar = 0;
+  ChainSync cs = chain_sync_begin(bar, &a.st[c], W, m_sh + 1);
   // The deferred strips of the previous block (its own rows, after the chain barrier), with
   // that block's Mx kept in Mxp and its Z tiles still in the workspace.  Each is applied by
   // its owner workgroup; while wave 0 runs the next serial phase the strips go to waves 1 ..
@@ -683,7 +717,7 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
   for (int b0 = 0; b0 < N; b0 += K) {
     const int kb = min(K, N - b0);
     if constexpr (K >= 16) {
-      if (pending) chain_barrier(bar, ++nbar * (unsigned)W, &a.st[c], broken);
+      if (pending) chain_barrier(cs);
     }
     if (threadIdx.x < kb) {
       const int i = b0 + threadIdx.x;
@@ -909,19 +943,19 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
   }
   if constexpr (K >= 16) {
     if (pending) {
-      chain_barrier(bar, ++nbar * (unsigned)W, &a.st[c], broken);
+      chain_barrier(cs);
       deferred_strips(false);
       __syncthreads();
     }
   }
   // On slice 0 every workgroup has read the sweep counter and (when measuring) every strip is
   // final before workgroup 0 measures and advances the counter.
-  if (W > 1 && l == 0) chain_barrier(bar, ++nbar * (unsigned)W, &a.st[c], broken);
+  if (W > 1 && l == 0) chain_barrier(cs);
   if (w == 0) {
     for (int i = threadIdx.x; i < N; i += blockDim.x) fld[i] = fl[i];
     slice_epilogue<T>(a, ch, c, l, G, nacc, underrun, bad, red);
   }
-  if (W > 1) chain_exit(bar, W);
+  if (W > 1) chain_exit(cs);
 }
 
 // ---- determinant mode ------------------------------------------------------------------------
@@ -1155,10 +1189,10 @@ __global__ __launch_bounds__(1024) void k_flip_rank1_split(FlipArgs a) {
   double* xv = prow + 2 * N;       // [2][nr] G[r, i] of my rows
   double* red = xv + 2 * nr;       // blockDim (epilogue)
   double* coef = red + blockDim.x;  // [2] update coefficients of both spins
-  int8_t* fl = reinterpret_cast<int8_t*>(coef + 2);  // [N] the slice's field
-  unsigned* bar = a.bar + 2 * c;
-  unsigned nbar = 0;
-  bool broken = false, underrun = false, bad = false;
+  int* bflag = reinterpret_cast<int*>(coef + 2);  // [2] the barrier's flag
+  int8_t* fl = reinterpret_cast<int8_t*>(bflag + 2);  // [N] the slice's field
+  ChainSync cs = chain_sync_begin(a.bar + 4 * c, &a.st[c], W, bflag);
+  bool underrun = false, bad = false;
   const unsigned long long sweep = a.st[c].sweep;
   unsigned long long nacc = 0;
   int pend = -1;  // site whose updated row waits in prow (uniform)
@@ -1203,7 +1237,7 @@ __global__ __launch_bounds__(1024) void k_flip_rank1_split(FlipArgs a) {
       const double de = threadIdx.x == 0 ? d_up : d_dn;
       coef[threadIdx.x] = al / de;
     }
-    chain_sync_reads(bar, ++nbar * (unsigned)W, &a.st[c], broken);
+    chain_sync_reads(cs);
     // G[r, :] -= c (x_r (e_i - G[i, :])): each wavefront takes RB rows at a time, lanes over
     // 256-column chunks, with every load of a batch issued before its arithmetic (the rows
     // stream from memory; one load at a time would leave the update latency-bound)
@@ -1249,7 +1283,7 @@ __global__ __launch_bounds__(1024) void k_flip_rank1_split(FlipArgs a) {
     pend = i;
     if (threadIdx.x == 0) fl[i] = (int8_t)(-s);
     ++nacc;
-    chain_barrier(bar, ++nbar * (unsigned)W, &a.st[c], broken);
+    chain_barrier(cs);
   }
   if (pend >= r0 && pend < r1) {
     for (int j = threadIdx.x; j < 2 * N; j += blockDim.x) {
@@ -1258,17 +1292,17 @@ __global__ __launch_bounds__(1024) void k_flip_rank1_split(FlipArgs a) {
     }
   }
   // every row final (and the sweep counter read by every workgroup) before workgroup 0 measures
-  if (l == 0) chain_barrier(bar, ++nbar * (unsigned)W, &a.st[c], broken);
+  if (l == 0) chain_barrier(cs);
   if (w == 0) {
     for (int i = threadIdx.x; i < N; i += blockDim.x) fld[i] = fl[i];
     slice_epilogue<T>(a, ch, c, l, G, nacc, underrun, bad, red);
   }
-  chain_exit(bar, W);
+  chain_exit(cs);
 }
 
 size_t flip_rank1_split_shmem(int nmax, int threads, int groups) {
   const size_t nr = ((size_t)nmax + groups - 1) / groups;
-  return sizeof(double) * (4 * (size_t)nmax + 2 * nr + threads + 2) + (size_t)nmax;
+  return sizeof(double) * (4 * (size_t)nmax + 2 * nr + threads + 3) + (size_t)nmax;
 }
 
 size_t det_sweep_shmem(int nmax, int lmax) {

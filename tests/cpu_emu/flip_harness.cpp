@@ -61,7 +61,7 @@ int main(int argc, char** argv) {
     }
   }
   std::memset(st.data(), 0, sizeof(ChainState) * nch);
-  std::vector<unsigned> bar(2 * nch, 0u);
+  std::vector<unsigned> bar(4 * nch, 0u);
   FlipArgs a{chains.data(), st.data(), t, measure, 1, 1, K, groups, bar.data(), nch, fp32 == 2 ? 1 : 0};
   if (K > 1 && (threads > flip_delayed_threads_limit(K) || threads < 256)) {
     std::fprintf(stderr, "threads above the delayed kernel's launch bound %d\n", flip_delayed_threads_limit(K));

@@ -114,6 +114,9 @@ inline void emu_wave_sync() { emu_waves[threadIdx.x >> 6].bar->arrive_and_wait()
 #define __builtin_amdgcn_fence(order, scope) ((void)0)
 #define __builtin_amdgcn_sched_barrier(mask) ((void)0)
 #define __builtin_amdgcn_s_waitcnt(mask) ((void)0)
+// Hardware register reads: XCC_ID (id 20) as the round-robin placement of block b on XCC b mod 8.
+inline int emu_getreg(int enc) { return (enc & 63) == 20 ? (int)(blockIdx.x % 8) : 0; }
+#define __builtin_amdgcn_s_getreg(enc) emu_getreg(enc)
 
 // Cross-lane exchange within a wavefront.
 inline double __shfl_xor(double v, int mask, int width = 64) {
