@@ -126,7 +126,7 @@ def cpu_model():
 
 # Bounded samples per process (whole slices of one chain of the workload, at the reference's
 # ~20 upd/s per core the faithful form needs ~13 s for one 16x16 slice).
-CPU_SAMPLES = {"faithful": 1, "outer": 16, "dqmc": 8}
+CPU_SAMPLES = {"faithful": 2, "outer": 64, "dqmc": 32}
 
 
 def cpu_baseline(cfg, procs=None):
