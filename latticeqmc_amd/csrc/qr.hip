@@ -737,6 +737,61 @@ __device__ inline void qo_apply(qo_acc_t (&acc)[16], const double* vbuf, const d
   }
 }
 
+// Q formation for N <= 256 with the column block held in registers: one 256-thread workgroup
+// per 64-column block of Q, wavefront w holding the 16 columns c0 + 16 w .. in the accumulator
+// layout of the FP64 matrix cores (all rows).  From the identity, the panels' block reflectors
+// H_P = I - V T V^T are applied from the last to the first (Q = H_0 ... H_{NS-1}): each panel's
+// V (explicit, unit diagonal, from the reflectors below the diagonal of A) and T (stored
+// transposed, since qo_apply applies I - V T_buf^T V^T) go to LDS once per workgroup, and a
+// wavefront skips the panels right of its columns, which leave them unchanged.  Q is written
+// once, at the end (the blocked k_qr_formq reads and writes its column block per panel).
+__global__ __launch_bounds__(QF_THREADS, 2) void k_qr_formq_onchip(const QrTask* __restrict__ tasks, int nblk) {
+  const QrTask tk = tasks[blockIdx.x / nblk];
+  const int N = tk.N;
+  const int c0 = (blockIdx.x % nblk) * QF_COLS;
+  if (c0 >= N) return;
+  const int c1 = min(N, c0 + QF_COLS);
+  const int NS = (N + 15) >> 4;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  LQMC_DYNAMIC_LDS(double, sm);
+  double* vbuf = sm;                      // [256][QO_PS] V of the panel
+  double* tbuf = vbuf + QO_NMAX * QO_PS;  // [16][16] T^T of the panel
+  const int cs = (c0 >> 4) + wave;        // my strip's column tile
+  const int c = (cs << 4) + li;           // my lane's column
+  qo_acc_t acc[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[t][v] = (16 * t + lk + 4 * v) == c ? 1.0 : 0.0;
+  for (int P = (c1 - 1) >> 4; P >= 0; --P) {
+    const int j0 = P << 4, w = min(16, N - j0);
+    __syncthreads();  // every wavefront is done with the previous panel's buffers
+    for (int e = tid; e < (QO_NMAX - j0) * 16; e += QF_THREADS) {
+      const int r = j0 + (e >> 4), cc = e & 15, jc = j0 + cc;
+      const bool in = r < N && cc < w;
+      const double a = ldg0(gptr(tk.A), (size_t)r * N + jc, in && r > jc);
+      vbuf[r * QO_PS + cc] = in ? (r > jc ? a : (r == jc ? 1.0 : 0.0)) : 0.0;
+    }
+    for (int e = tid; e < 256; e += QF_THREADS) {
+      const int i = e >> 4, j = e & 15;
+      tbuf[e] = (i < w && j < w) ? tk.Tt[(size_t)P * 256 + j * 16 + i] : 0.0;
+    }
+    __syncthreads();
+    if (cs >= P && cs < NS) qo_apply(acc, vbuf, tbuf, NS, P);
+  }
+  if (c < c1)
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = 16 * t + lk + 4 * v;
+        if (r < N) tk.Q[(size_t)r * N + c] = acc[t][v];
+      }
+}
+
+size_t qr_formq_onchip_shmem() { return sizeof(double) * ((size_t)QO_NMAX * QO_PS + 256); }
+
 struct QoShared {
   double *sbuf, *vbuf, *tbuf, *ssc, *red, *dbuf, *taus;
   const int* perm;
@@ -1033,7 +1088,10 @@ void launch_qr_blocked(const QrTask* d, int n, int nmax, ChainState* st, unsigne
     hipLaunchKernelGGL(k_qr_blocked, dim3(n), dim3(QR_THREADS), qr_blocked_shmem(nmax), s, d, st);
   }
   const int nblk = (nmax + QF_COLS - 1) / QF_COLS;
-  hipLaunchKernelGGL(k_qr_formq, dim3(n * nblk), dim3(QF_THREADS), qr_formq_shmem(nmax), s, d, nblk);
+  if (nmax <= QO_NMAX)
+    hipLaunchKernelGGL(k_qr_formq_onchip, dim3(n * nblk), dim3(QF_THREADS), qr_formq_onchip_shmem(), s, d, nblk);
+  else
+    hipLaunchKernelGGL(k_qr_formq, dim3(n * nblk), dim3(QF_THREADS), qr_formq_shmem(nmax), s, d, nblk);
 }
 
 void launch_trsm(const TrsmTask* d, int n, int nmax, ChainState* st, hipStream_t s) {

@@ -20,7 +20,7 @@
 #define __global__
 #define __device__
 #define __host__
-#define __launch_bounds__(x)
+#define __launch_bounds__(...)
 #define __shared__ static
 #define __restrict__
 

@@ -4,7 +4,8 @@
 // order requested), Q^T Q = I and the permuted triangular solve.
 // usage: qr_harness N decades seed [near|onchip]
 // onchip: the factorisation by k_qr_onchip (N <= 256: one workgroup per 128 columns, both at
-// once) instead of k_qr_blocked; its hand-over counters must be back at zero afterwards.
+// once) instead of k_qr_blocked, and Q formed by k_qr_formq_onchip; the hand-over counters
+// must be back at zero afterwards.
 #include "hip_cpu_shim.h"
 #include "../../latticeqmc_amd/csrc/qr.hip"
 
@@ -60,7 +61,11 @@ int main(int argc, char** argv) {
   std::vector<double> Wv(W0);
   QrTask qv{Wv.data(), Q, D, A, nullptr, Tt, N, 0};
   factor(&qv, &st);
-  emu_launch(nblk, QF_THREADS, qr_formq_shmem(N), [&] { k_qr_formq(&qv, nblk); });
+  auto formq = [&](QrTask* task) {
+    if (onchip) emu_launch(nblk, QF_THREADS, qr_formq_onchip_shmem(), [&] { k_qr_formq_onchip(task, nblk); });
+    else emu_launch(nblk, QF_THREADS, qr_formq_shmem(N), [&] { k_qr_formq(task, nblk); });
+  };
+  formq(&qv);
   double recv = 0.0;
   for (int c = 0; c < N; ++c) {
     double cn = 0.0, en = 0.0;
@@ -77,7 +82,7 @@ int main(int argc, char** argv) {
   std::fill(Q, Q + n2, 0.0);
   QrTask qt{W, Q, D, A, perm, Tt, N, 0};
   factor(&qt, &st);
-  emu_launch(nblk, QF_THREADS, qr_formq_shmem(N), [&] { k_qr_formq(&qt, nblk); });
+  formq(&qt);
   // reconstruction, column-relative: |W - Q D R'|_col / |W|_col
   double rec = 0.0, orth = 0.0;
   for (int c = 0; c < N; ++c) {
