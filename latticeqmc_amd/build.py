@@ -34,13 +34,19 @@ def flags():
             "-I", os.path.join(ROOT, "include"), "-Wno-unused-command-line-argument"]
 
 
+# Per-source extra flags.  gemm.hip: keep the matrix-core accumulators in VGPRs (the default
+# AGPR form copied all 32 accumulator registers between the two files on every k-tile of the
+# GEMM loop).
+EXTRA = {"gemm.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     cc = hipcc()
 
     def compile_one(src):
         obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-        cmd = [cc, *flags(), "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [cc, *flags(), *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")
