@@ -44,12 +44,13 @@ __device__ inline double scale_val(int kind, const double* s, int idx) {
 }
 
 constexpr int BM = 64, BN = 64, BK = 16, PAD = 4;
+constexpr int GEMM_KMAX = 1024;  // longest inner dimension (the chains' N <= 1024)
 
 // One 64x64 tile of C = alpha diag(rs) op(A) diag(ks) op(B) diag(cs) + beta C for one
 // transpose combination (TA, TB known at compile time, so every operand load is one load
-// through a computed index).  The k-scaling factors of each k-tile are formed once, by 16
-// threads, into LDS and applied as the A tile is stored; the row and column factors of
-// the epilogue are formed once per lane.
+// through a computed index).  The k-scaling factors of the whole K range are formed once, at
+// the start, into LDS and applied as the A tile is stored; the row and column factors of the
+// epilogue are formed once per lane.
 //  FULL: every tile of the task lies inside C and K is a multiple of BK, so the operand
 //  loads need no bounds checks.
 template <typename T, int TA, int TB, bool FULL>
@@ -57,8 +58,8 @@ __device__ void gemm_tile(const GemmTask& tk, int m0, int n0, T (*As)[BM + PAD],
                           double* ksv) {
   using acc_t = typename Mfma<T>::acc_t;
   const int M = tk.M, N = tk.N, K = tk.K;
-  const T* __restrict__ A = static_cast<const T*>(tk.A);
-  const T* __restrict__ B = static_cast<const T*>(tk.B);
+  const LQMC_GLOBAL T* __restrict__ A = gptr(static_cast<const T*>(tk.A));
+  const LQMC_GLOBAL T* __restrict__ B = gptr(static_cast<const T*>(tk.B));
   T* __restrict__ C = static_cast<T*>(tk.C);
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -70,7 +71,6 @@ __device__ void gemm_tile(const GemmTask& tk, int m0, int n0, T (*As)[BM + PAD],
   // A: non-trans: row m = tid/4, k = (tid%4)*4 + q ; trans: k = tid/16, m = (tid%16)*4 + q
   // B: non-trans: k = tid/16, n = (tid%16)*4 + q ; trans: n = tid/4, k = (tid%4)*4 + q
   T ra[4], rb[4];
-  double kf = 1.0;  // threads < 16: the k-scaling factor of row k0 + tid of the next tile
   auto load_tiles = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -83,14 +83,13 @@ __device__ void gemm_tile(const GemmTask& tk, int m0, int n0, T (*As)[BM + PAD],
       const size_t ib = TB ? (size_t)n * tk.ldb + kb : (size_t)kb * tk.ldb + n;
       rb[q] = (FULL || (n < N && kb < K)) ? B[ib] : T(0);
     }
-    if (kscale && tid < BK) kf = k0 + tid < K ? scale_val(tk.ks_kind, tk.ks, k0 + tid) : 1.0;
   };
-  auto store_tiles = [&]() {
-    // ksv was written before the barrier that precedes this call
+  auto store_tiles = [&](int k0) {
+    // ksv holds the k-scaling factors of the whole K range (formed before the first barrier)
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int ka = TA ? (tid >> 4) : ((tid & 3) << 2) + q;
-      const T av = kscale ? ra[q] * T(ksv[ka]) : ra[q];
+      const T av = kscale ? ra[q] * T(ksv[k0 + ka]) : ra[q];
       if (!TA) As[((tid & 3) << 2) + q][tid >> 2] = av;
       else As[tid >> 4][((tid & 15) << 2) + q] = av;
       if (!TB) Bs[tid >> 4][((tid & 15) << 2) + q] = rb[q];
@@ -106,9 +105,10 @@ __device__ void gemm_tile(const GemmTask& tk, int m0, int n0, T (*As)[BM + PAD],
 
   const int nk = (K + BK - 1) / BK;
   load_tiles(0);
-  if (kscale && tid < BK) ksv[tid] = kf;
+  if (kscale)
+    for (int k = tid; k < nk * BK; k += 256) ksv[k] = k < K ? scale_val(tk.ks_kind, tk.ks, k) : 1.0;
   __syncthreads();
-  store_tiles();
+  store_tiles(0);
   __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
     if (kt + 1 < nk) load_tiles((kt + 1) * BK);
@@ -124,11 +124,9 @@ __device__ void gemm_tile(const GemmTask& tk, int m0, int n0, T (*As)[BM + PAD],
       acc[1][0] = Mfma<T>::mma(a1, b0, acc[1][0]);
       acc[1][1] = Mfma<T>::mma(a1, b1, acc[1][1]);
     }
-    // ksv was last read by the store phase before this tile's products (a barrier apart)
-    if (kt + 1 < nk && kscale && tid < BK) ksv[tid] = kf;
-    __syncthreads();  // every read of As and Bs for this tile is done; ksv is visible
+    __syncthreads();  // every read of As and Bs for this tile is done
     if (kt + 1 < nk) {
-      store_tiles();
+      store_tiles((kt + 1) * BK);
       __syncthreads();
     }
   }
@@ -186,7 +184,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
   const int n0 = (local % tk.tiles_n) * BN;
   __shared__ T As[BK][BM + PAD];
   __shared__ T Bs[BK][BN + PAD];
-  __shared__ double ksv[BK];
+  __shared__ double ksv[GEMM_KMAX];
   gemm_tile<T, TA, TB, FULL>(tk, m0, n0, As, Bs, ksv);
 }
 
