@@ -792,6 +792,93 @@ __global__ __launch_bounds__(QF_THREADS, 2) void k_qr_formq_onchip(const QrTask*
 
 size_t qr_formq_onchip_shmem() { return sizeof(double) * ((size_t)QO_NMAX * QO_PS + 256); }
 
+// R' Y' = Z, Y[perm[r]] = Y'[r] for N <= 256 with the solution held in registers: one
+// 256-thread workgroup per 64-column block, wavefront w holding Y'[:, c0 + 16 w ..] in the
+// accumulator layout (row 16 t + l/16 + 4 v, column l%16 of lane l).  Row blocks of 16 from the
+// bottom: the block's band of R' goes to LDS, each wavefront forms Z[blk] - R'[blk, below]
+// Y'[below] on the matrix cores from its registers (the accumulator layout of Y' is the operand
+// layout of the product), the 16 x 16 diagonal systems are solved one lane per column through
+// LDS, and the block joins the registers.  Y is written once, permuted, at the end (the blocked
+// k_trsm_perm re-reads the solution below each block from global memory).
+constexpr int TO_LD = QO_NMAX + 1;  // LDS row stride of the R' band
+
+__global__ __launch_bounds__(QF_THREADS, 2) void k_trsm_onchip(const TrsmTask* __restrict__ tasks, int nblk,
+                                                               ChainState* st) {
+  const TrsmTask tk = tasks[blockIdx.x / nblk];
+  const int N = tk.N;
+  const int c0 = (blockIdx.x % nblk) * QF_COLS;
+  if (c0 >= N) return;
+  const int c1 = min(N, c0 + QF_COLS);
+  const int NS = (N + 15) >> 4;
+  const LQMC_GLOBAL double* Rg = gptr(tk.V);
+  const LQMC_GLOBAL double* Zg = gptr(tk.Z);
+  LQMC_DYNAMIC_LDS(double, sm);
+  double* band = sm;                 // [16][TO_LD] rows i0 .. i0+15 of R' (columns i0 .. N-1)
+  double* blk = band + 16 * TO_LD;   // [16][QF_COLS] the diagonal block's right-hand sides
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int c = c0 + wave * 16 + li;
+  const bool cok = c < c1;
+  bool zero = false;
+  qo_acc_t y[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) y[t] = qo_acc_t{0.0, 0.0, 0.0, 0.0};
+  for (int i = NS - 1; i >= 0; --i) {
+    // the band's columns i0 .. 16 NS - 1, zero past N (the products below read whole 16-column
+    // tiles, whose rows of Y' past N are zero: stale LDS there could hold a NaN)
+    const int i0 = i << 4, b = min(16, N - i0), nc = 16 * NS - i0;
+    __syncthreads();  // the previous block's reads of band and blk are done
+    for (int e = tid; e < 16 * nc; e += QF_THREADS) {
+      const int ii = e / nc, cc = i0 + e % nc;
+      band[ii * TO_LD + cc] = ldg0(Rg, (size_t)(i0 + ii) * N + cc, ii < b && cc < N);
+    }
+    qr_v4d acc;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = lk + 4 * v;
+      acc[v] = ldg0(Zg, (size_t)(i0 + row) * N + c, row < b && cok);
+    }
+    __syncthreads();
+    // acc -= R'[blk, j] Y'_j for the blocks j below (A lane l: R'[i0 + l%16][16 j + l/16 + 4 v])
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      if (j <= i || j >= NS) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc = qr_mma(-band[li * TO_LD + 16 * j + lk + 4 * v], y[j][v], acc);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) blk[(lk + 4 * v) * QF_COLS + wave * 16 + li] = acc[v];
+    __syncthreads();
+    if (tid < QF_COLS) {
+      for (int ii = b - 1; ii >= 0; --ii) {
+        double sv = blk[ii * QF_COLS + tid];
+        for (int kk = ii + 1; kk < b; ++kk) sv -= band[ii * TO_LD + i0 + kk] * blk[kk * QF_COLS + tid];
+        const double d = band[ii * TO_LD + i0 + ii];
+        if (d == 0.0) zero = true;
+        blk[ii * QF_COLS + tid] = sv / d;
+      }
+    }
+    __syncthreads();
+    qr_v4d sol;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sol[v] = blk[(lk + 4 * v) * QF_COLS + wave * 16 + li];
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+      if (t == i) y[t] = sol;
+  }
+  if (cok)
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = 16 * t + lk + 4 * v;
+        if (r < N) tk.Y[(size_t)tk.perm[r] * N + c] = y[t][v];
+      }
+  if (zero && st) atomicOr(&st[tk.chain].flags, 2u);
+}
+
+size_t trsm_onchip_shmem() { return sizeof(double) * (16 * (size_t)TO_LD + 16 * QF_COLS); }
+
 struct QoShared {
   double *sbuf, *vbuf, *tbuf, *ssc, *red, *dbuf, *taus;
   const int* perm;
@@ -1097,7 +1184,10 @@ void launch_qr_blocked(const QrTask* d, int n, int nmax, ChainState* st, unsigne
 void launch_trsm(const TrsmTask* d, int n, int nmax, ChainState* st, hipStream_t s) {
   if (n <= 0) return;
   const int nblk = (nmax + QF_COLS - 1) / QF_COLS;
-  hipLaunchKernelGGL(k_trsm_perm, dim3(n * nblk), dim3(QF_THREADS), trsm_shmem(nmax), s, d, nblk, st);
+  if (nmax <= QO_NMAX)
+    hipLaunchKernelGGL(k_trsm_onchip, dim3(n * nblk), dim3(QF_THREADS), trsm_onchip_shmem(), s, d, nblk, st);
+  else
+    hipLaunchKernelGGL(k_trsm_perm, dim3(n * nblk), dim3(QF_THREADS), trsm_shmem(nmax), s, d, nblk, st);
 }
 
 }  // namespace lqmc

@@ -165,7 +165,7 @@ struct EmuBlock {
   std::vector<EmuWave> waves;
   EmuBlock(int threads, size_t lds_bytes) : bar(threads), waves((threads + 63) / 64) {
     lds = static_cast<char*>(std::malloc(lds_bytes ? lds_bytes : 1));
-    std::memset(lds, 0xA5, lds_bytes);
+    std::memset(lds, 0xFF, lds_bytes);  // NaN as doubles: a read of unwritten LDS poisons the result
     for (auto& w : waves) w.bar = new std::barrier<>(64);
   }
   ~EmuBlock() {

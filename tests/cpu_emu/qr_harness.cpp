@@ -4,8 +4,8 @@
 // order requested), Q^T Q = I and the permuted triangular solve.
 // usage: qr_harness N decades seed [near|onchip]
 // onchip: the factorisation by k_qr_onchip (N <= 256: one workgroup per 128 columns, both at
-// once) instead of k_qr_blocked, and Q formed by k_qr_formq_onchip; the hand-over counters
-// must be back at zero afterwards.
+// once) instead of k_qr_blocked, Q formed by k_qr_formq_onchip and the solve by k_trsm_onchip;
+// the hand-over counters must be back at zero afterwards.
 #include "hip_cpu_shim.h"
 #include "../../latticeqmc_amd/csrc/qr.hip"
 
@@ -117,7 +117,8 @@ int main(int argc, char** argv) {
   std::vector<double> Z0(n2);
   for (size_t e = 0; e < n2; ++e) Z0[e] = Z[e] = g(rng);
   TrsmTask tt{W, perm, Z, Y, N, 0};
-  emu_launch(nblk, QF_THREADS, trsm_shmem(N), [&] { k_trsm_perm(&tt, nblk, &st); });
+  if (onchip) emu_launch(nblk, QF_THREADS, trsm_onchip_shmem(), [&] { k_trsm_onchip(&tt, nblk, &st); });
+  else emu_launch(nblk, QF_THREADS, trsm_shmem(N), [&] { k_trsm_perm(&tt, nblk, &st); });
   double res = 0.0, zn = 0.0;
   for (int r = 0; r < N; ++r)
     for (int c = 0; c < N; ++c) {
@@ -136,8 +137,14 @@ int main(int argc, char** argv) {
   // nearly dependent pairs split over two panels leave R' ill-conditioned (max|R'| ~ 1e7): there
   // the solve is held to its backward error instead of the residual and the |R'| bound
   const bool solve_ok = near ? bwd < 1e-15 : (vmax < 1e3 && res / zn < 1e-9);
+  // every output finite (the maxima above ignore NaN; the emulated LDS starts as NaN)
+  size_t nonfinite = 0;
+  for (size_t e = 0; e < n2; ++e)
+    nonfinite += !std::isfinite(Q[e]) + !std::isfinite(W[e]) + !std::isfinite(Y[e]) + !std::isfinite(Wv[e]);
+  for (int j = 0; j < N; ++j) nonfinite += !std::isfinite(D[j]);
+  if (nonfinite) std::printf("non-finite outputs: %zu\n", nonfinite);
   const bool ok = recv < 1e-12 && rec < 1e-12 && orth < 1e-12 && low == 0.0 && dia < 1e-12 && solve_ok &&
-                  st.flags == 0 && hand[0] == 0u && hand[1] == 0u;
+                  nonfinite == 0 && st.flags == 0 && hand[0] == 0u && hand[1] == 0u;
   std::printf(ok ? "ok\n" : "FAIL\n");
   return ok ? 0 : 1;
 }
