@@ -7,6 +7,9 @@
 // Global address space: loads through such pointers are global_load (vector-memory
 // counter only) rather than flat loads, which also hold the LDS counter.
 #define LQMC_GLOBAL __attribute__((address_space(1)))
+// The value of x (in an SGPR: "s", a VGPR: "v") passed through an empty asm: expressions
+// formed from it afterwards are new to the compiler and are not merged with earlier ones.
+#define LQMC_OPAQUE(x, reg) asm volatile("" : "+" reg(x))
 #endif
 #include <stdint.h>
 

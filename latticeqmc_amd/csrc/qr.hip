@@ -994,14 +994,21 @@ __device__ void qo_factor(qo_acc_t (&acc)[16], const QrTask& tk, const QoShared&
     }
     qr_wave_sync();
   }
-  // D^-1 R of the strip's columns over W: V'' = D^-1 R P^T (columns to their original
-  // places), or R' = D^-1 R with the pivot order
-  for (int e = lane; e < N * 16; e += 64) {
-    const int r = e >> 4, c = e & 15, jc = j0 + c;
-    if (c >= w) continue;
-    const double y = r <= jc ? sbuf[r * QO_PS + c] / sh.dbuf[r] : 0.0;
-    tk.W[(size_t)r * N + (tk.perm ? jc : sh.perm[jc])] = y;
-  }
+}
+
+// R of strip S (rows 0..16S+15, upper triangle of the panel block) from sbuf back to the
+// accumulator layout; zero below.  Runs after the panel's release barrier, while the next
+// wavefront applies the panel and before it overwrites sbuf.
+__device__ inline void qo_keep_r(qo_acc_t (&acc)[16], const QoShared& sh, int S) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const int jc = 16 * S + li;
+#pragma unroll
+  for (int t = 0; t < 16; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = 16 * t + lk + 4 * v;
+      acc[t][v] = ld0(sh.sbuf, (size_t)r * QO_PS + li, r <= jc);
+    }
 }
 
 __global__ __launch_bounds__(QO_THREADS) void k_qr_onchip(const QrTask* __restrict__ tasks, int ntasks, int G,
@@ -1119,7 +1126,10 @@ __global__ __launch_bounds__(QO_THREADS) void k_qr_onchip(const QrTask* __restri
   // 2. my panel, 3. the panels right of it
   if (have) {
     qo_factor(acc, tk, sh, N, S, min(16, N - 16 * S), hd, part == 0, st, broken, zero_pivot);
-    for (int P = S; P < P1; ++P) {
+    __syncthreads();
+    qo_keep_r(acc, sh, S);
+    __syncthreads();
+    for (int P = S + 1; P < P1; ++P) {
       __syncthreads();
       __syncthreads();
     }
@@ -1128,6 +1138,27 @@ __global__ __launch_bounds__(QO_THREADS) void k_qr_onchip(const QrTask* __restri
   if (hd && tid == 0) atomicExch(hd + (part == 1 ? 0 : 1), 0u);  // counters back to zero for the next launch
   if (part == 0 && tk.perm)
     for (int i = tid; i < N; i += QO_THREADS) tk.perm[i] = perm[i];
+  // 4. D^-1 R of my strip over W: V'' = D^-1 R P^T (columns to their original places), or
+  //    R' = D^-1 R with the pivot order.  Off the panel chain: every |diag R| this strip
+  //    needs (rows <= its last column) is in dbuf by now.
+  if (have) {
+    // N and the lane's row made opaque, so the compiler forms these 64 row offsets and
+    // bounds here instead of keeping those of the strip load live all kernel (128 VGPRs)
+    int Nw = N, lkw = lk;
+    LQMC_OPAQUE(Nw, "s");
+    LQMC_OPAQUE(lkw, "v");
+    const int jc = 16 * S + li;
+    if (jc < Nw) {
+      const int oc = tk.perm ? jc : perm[jc];
+#pragma unroll
+      for (int t = 0; t < 16; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = 16 * t + lkw + 4 * v;
+          if (r < Nw) tk.W[(size_t)r * Nw + oc] = r <= jc ? acc[t][v] / sh.dbuf[r] : 0.0;
+        }
+    }
+  }
   if (st) {
     if (zero_pivot && lane == 0) atomicOr(&st[tk.chain].flags, 2u);
     if (bad) atomicOr(&st[tk.chain].flags, 8u);

@@ -31,6 +31,7 @@ inline thread_local char* emu_lds_buf = nullptr;
 inline thread_local std::barrier<>* emu_block_barrier = nullptr;
 #define LQMC_DYNAMIC_LDS(type, name) type* name = reinterpret_cast<type*>(emu_lds_buf)
 #define LQMC_GLOBAL
+#define LQMC_OPAQUE(x, reg) ((void)(x))
 inline void __syncthreads() { emu_block_barrier->arrive_and_wait(); }
 
 using std::fabs; using std::fmax; using std::fmin; using std::isfinite; using std::max; using std::min; using std::sqrt;
