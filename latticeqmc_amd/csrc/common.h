@@ -18,6 +18,16 @@ __device__ inline LQMC_GLOBAL T* gptr(T* p) {
   return (LQMC_GLOBAL T*)p;
 }
 
+// Block index regrouped by XCD.  Workgroups are dispatched to the eight XCDs round-robin
+// (block b on XCD b % 8), each XCD with its own L2.  The returned index runs consecutively
+// over the blocks of one XCD, so neighbouring indices (the tiles or column blocks of one
+// matrix, which read the same operand panels) share an L2; blocks past the last multiple of
+// 8 keep their index.
+__device__ inline int xcd_grouped_block() {
+  const int q8 = (int)gridDim.x >> 3, b = (int)blockIdx.x;
+  return b < (q8 << 3) ? (b & 7) * q8 + (b >> 3) : b;
+}
+
 // p[ok ? i : 0], then 0 where !ok: the load is unconditional (no branch, so loads of
 // a loop body can all be in flight together) and always addresses valid memory.
 template <typename T>

@@ -172,7 +172,8 @@ __device__ void gemm_tile(const GemmTask& tk, int m0, int n0, T (*As)[BM + PAD],
 // only, so each kernel keeps the registers of a single tile body.
 template <typename T, int TA, int TB, bool FULL>
 __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks, int ntasks) {
-  const int tile = blockIdx.x;
+  // the tiles of one product (sharing the operands' row and column panels) on one XCD
+  const int tile = xcd_grouped_block();
   int lo = 0, hi = ntasks - 1;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;

@@ -524,9 +524,10 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_blocked(const QrTask* __restr
 // rows >= j only, so the identity columns c0..c1-1 are touched by panels j0 <= c1-1, and
 // panel j0 changes rows and columns >= j0 only.
 __global__ __launch_bounds__(QF_THREADS) void k_qr_formq(const QrTask* __restrict__ tasks, int nblk) {
-  const QrTask tk = tasks[blockIdx.x / nblk];
+  const int gb = xcd_grouped_block();  // a matrix's column blocks on one XCD
+  const QrTask tk = tasks[gb / nblk];
   const int N = tk.N;
-  const int c0 = (blockIdx.x % nblk) * QF_COLS;
+  const int c0 = (gb % nblk) * QF_COLS;
   if (c0 >= N) return;
   const int c1 = min(N, c0 + QF_COLS), cw = c1 - c0;
   const int nb = qr_panel_width(N);
@@ -573,9 +574,10 @@ __global__ __launch_bounds__(QF_THREADS) void k_qr_formq(const QrTask* __restric
 // column solves the 16 x 16 diagonal block.
 __global__ __launch_bounds__(QF_THREADS) void k_trsm_perm(const TrsmTask* __restrict__ tasks, int nblk,
                                                           ChainState* st) {
-  const TrsmTask tk = tasks[blockIdx.x / nblk];
+  const int gb = xcd_grouped_block();  // a matrix's column blocks on one XCD
+  const TrsmTask tk = tasks[gb / nblk];
   const int N = tk.N;
-  const int c0 = (blockIdx.x % nblk) * QF_COLS;
+  const int c0 = (gb % nblk) * QF_COLS;
   if (c0 >= N) return;
   const int c1 = min(N, c0 + QF_COLS);
   const double* __restrict__ R = tk.V;
@@ -746,9 +748,10 @@ __device__ inline void qo_apply(qo_acc_t (&acc)[16], const double* vbuf, const d
 // wavefront skips the panels right of its columns, which leave them unchanged.  Q is written
 // once, at the end (the blocked k_qr_formq reads and writes its column block per panel).
 __global__ __launch_bounds__(QF_THREADS, 2) void k_qr_formq_onchip(const QrTask* __restrict__ tasks, int nblk) {
-  const QrTask tk = tasks[blockIdx.x / nblk];
+  const int gb = xcd_grouped_block();  // a matrix's column blocks on one XCD
+  const QrTask tk = tasks[gb / nblk];
   const int N = tk.N;
-  const int c0 = (blockIdx.x % nblk) * QF_COLS;
+  const int c0 = (gb % nblk) * QF_COLS;
   if (c0 >= N) return;
   const int c1 = min(N, c0 + QF_COLS);
   const int NS = (N + 15) >> 4;
@@ -804,9 +807,10 @@ constexpr int TO_LD = QO_NMAX + 1;  // LDS row stride of the R' band
 
 __global__ __launch_bounds__(QF_THREADS, 2) void k_trsm_onchip(const TrsmTask* __restrict__ tasks, int nblk,
                                                                ChainState* st) {
-  const TrsmTask tk = tasks[blockIdx.x / nblk];
+  const int gb = xcd_grouped_block();  // a matrix's column blocks on one XCD
+  const TrsmTask tk = tasks[gb / nblk];
   const int N = tk.N;
-  const int c0 = (blockIdx.x % nblk) * QF_COLS;
+  const int c0 = (gb % nblk) * QF_COLS;
   if (c0 >= N) return;
   const int c1 = min(N, c0 + QF_COLS);
   const int NS = (N + 15) >> 4;
