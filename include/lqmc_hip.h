@@ -77,6 +77,12 @@ typedef struct {
                           raises LQMC_FLAG_PRECISION, so lqmc_sync fails (0 = 1e-3; < 0 = off).
                           The Metropolis ratios between recomputes read the propagated G, so a
                           large deviation (an FP32 form at low temperature) biases the chain. */
+  int32_t dense_ek;    /* 1: every product with E_K or E_K^-1 as a dense GEMM.  0 (default): where
+                          E_K = Ex (x) Ey and E_K^-1 = Exi (x) Eyi exactly (a hopping matrix that is
+                          a Kronecker sum, e.g. the square and cubic lattices; checked entry by
+                          entry to 1e-13 of max|E_K| in lqmc_set_chain, n1, n2 <= 16), the FP64
+                          products apply the factors: 2 N^2 (n1 + n2) flops in place of 2 N^3,
+                          equal to the dense product up to rounding. */
 } lqmc_options;
 
 typedef struct {
@@ -107,6 +113,11 @@ int lqmc_set_chain(lqmc_ctx* ctx, int c, const double* expk, const double* expk_
 /* Computes the initial Green's functions: Mode D the stabilised G(L-1)
  * (what _update_step's first inversion provides, lqmc/lqmc.py:303-307). */
 int lqmc_init(lqmc_ctx* ctx);
+
+/* The Kronecker split N = n1 x n2 used for chain c's E_K products (after lqmc_set_chain), or
+ * 0 x 0 where they run as dense GEMMs.  No counterpart in the reference, which multiplies
+ * by the dense expm(-dtau K) (lqmc/lqmc.py:93-117); it reports which path the products take. */
+int lqmc_get_ek_split(lqmc_ctx* ctx, int c, int32_t* n1, int32_t* n2);
 
 /* Uniforms for replay mode (the np.random.rand() draws of lqmc/lqmc.py:317):
  * u[s*L*N + t*N + i] for sweeps s = 0..n_sweeps-1 starting at the chain's next
@@ -183,7 +194,8 @@ int lqmc_get_observables(lqmc_ctx* ctx, int c, double* out, int max_rows, int* r
 /* Average device duration (ms) of the kernels of one kind over the sweeps run
  * since lqmc_profile(ctx, kind) was called, measured with HIP events on the
  * context stream (graphs are bypassed while profiling).  kind: 0 = flip,
- * 1 = GEMM, 2 = QR, 3 = solve; kind = -1 stops profiling. */
+ * 1 = GEMM, 2 = QR, 3 = solve, 4 = Kronecker-factored E_K products; kind = -1
+ * stops profiling. */
 int lqmc_profile(lqmc_ctx* ctx, int kind);
 int lqmc_profile_result(lqmc_ctx* ctx, double* avg_ms, int64_t* launches, double* work_units);
 

@@ -18,7 +18,7 @@ FLAG_NAMES = {1: "NaN/Inf ratio", 2: "zero pivot", 4: "replay uniforms exhausted
               16: "flip barrier timeout", 32: "recompute deviation above max_dg"}
 
 EXPORTS = (
-    "lqmc_create", "lqmc_set_chain", "lqmc_init", "lqmc_set_uniforms", "lqmc_sweep",
+    "lqmc_create", "lqmc_set_chain", "lqmc_init", "lqmc_get_ek_split", "lqmc_set_uniforms", "lqmc_sweep",
     "lqmc_sync", "lqmc_use_graph", "lqmc_get_green", "lqmc_get_green_dev", "lqmc_get_recompute_err", "lqmc_get_spin_corr", "lqmc_get_green_tau",
     "lqmc_get_field",
     "lqmc_get_stats", "lqmc_reset_stats", "lqmc_get_trace", "lqmc_get_observables",
@@ -32,7 +32,7 @@ class Options(ctypes.Structure):
                 ("n_stab", ctypes.c_int32), ("rng", ctypes.c_int32), ("trace", ctypes.c_int32),
                 ("obs_capacity", ctypes.c_int32), ("flip_threads", ctypes.c_int32),
                 ("flip_groups", ctypes.c_int32), ("time_displaced", ctypes.c_int32),
-                ("max_dg", ctypes.c_double)]
+                ("max_dg", ctypes.c_double), ("dense_ek", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -74,6 +74,7 @@ def load(path=None):
         "lqmc_create": (I, [P(V), I, I, i32p, i32p, P(Options)]),
         "lqmc_set_chain": (I, [V, I, f64p, f64p, f64p, D, P(ctypes.c_int8), ctypes.c_uint64]),
         "lqmc_init": (I, [V]),
+        "lqmc_get_ek_split": (I, [V, I, i32p, i32p]),
         "lqmc_set_uniforms": (I, [V, I, f64p, ctypes.c_int64]),
         "lqmc_sweep": (I, [V, I, I, V]),
         "lqmc_sync": (I, [V]),

@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(ROOT, "_build")
 LIB = os.path.join(HERE, "liblqmc_hip.so")
-SOURCES = ["engine.hip", "gemm.hip", "linalg.hip", "flip.hip", "qr.hip", "wrap.hip"]
+SOURCES = ["engine.hip", "gemm.hip", "kron.hip", "linalg.hip", "flip.hip", "qr.hip", "wrap.hip"]
 ARCH = os.environ.get("LQMC_OFFLOAD_ARCH", "gfx950")
 
 

@@ -87,6 +87,19 @@ struct ChainState {
 // Diagonal-scale kinds applied to GEMM rows (rs), inner index (ks) or columns (cs).
 enum ScaleKind : int { SC_NONE = 0, SC_PLAIN = 1, SC_INV = 2, SC_INV_BIG = 3, SC_SMALL = 4 };
 
+// Entry idx of a diagonal scale of the given kind (1 for SC_NONE).
+__device__ inline double scale_val(int kind, const double* s, int idx) {
+  if (kind == SC_NONE) return 1.0;
+  double v = s[idx];
+  switch (kind) {
+    case SC_PLAIN: return v;
+    case SC_INV: return 1.0 / v;
+    case SC_INV_BIG: return 1.0 / fmax(v, 1.0);
+    case SC_SMALL: return fmin(v, 1.0);
+    default: return 1.0;
+  }
+}
+
 // C = alpha * diag(rs) * op(A) * diag(ks) * op(B) * diag(cs) + beta * C
 struct GemmTask {
   const void* A;
@@ -103,6 +116,23 @@ struct GemmTask {
   int tile_base;    // first tile index of this task in the launch
   int pad;
   double alpha, beta;
+};
+
+// C = diag(rs) E diag(ks) X diag(cs) (left) or C = diag(rs) X diag(ks) E diag(cs) (right) for
+// a Kronecker-factored E = Ex (x) Ey (site i = a n2 + b; Ex n1 x n1, Ey n2 x n2, n1, n2 <= 16,
+// each stored as a 16 x 16 row-major block, zero outside).  X and C are N x N, N = n1 n2.
+struct KronTask {
+  const double* X;
+  double* C;
+  const double* Ex;
+  const double* Ey;
+  const double* rs;
+  const double* ks;
+  const double* cs;
+  int N, n1, n2;
+  int ldx, ldc;
+  int rs_kind, ks_kind, cs_kind;
+  int tile_base;  // first 16-column (left) or 16-row (right) tile of this task in the launch
 };
 
 // Pre-pivoted blocked Householder QR: W = Q diag(D) V'' with V'' written over W.

@@ -28,6 +28,8 @@
 
 namespace lqmc {
 void launch_gemm(int fp32, int cls, const GemmTask* d_tasks, int ntasks, int ntiles, hipStream_t s);
+void launch_kron(int right, const KronTask* d_tasks, int ntasks, int ntiles, hipStream_t s);
+int kron_tiles(int N);
 int gemm_class(const GemmTask& t);
 int gemm_tiles(int M, int N, int* tiles_n);
 void launch_qr_blocked(const QrTask* d, int n, int nmax, ChainState* st, unsigned* pub, int max_wg, hipStream_t s);
@@ -74,8 +76,8 @@ static int fail(int code, const std::string& msg) {
 
 namespace {
 
-enum OpKind { OP_GEMM = 0, OP_QR, OP_SOLVE, OP_FINISH, OP_FLIP, OP_EXPV, OP_IDENT, OP_TRSM, OP_DET };
-enum ProfKind { PK_FLIP = 0, PK_GEMM = 1, PK_QR = 2, PK_SOLVE = 3 };
+enum OpKind { OP_GEMM = 0, OP_QR, OP_SOLVE, OP_FINISH, OP_FLIP, OP_EXPV, OP_IDENT, OP_TRSM, OP_DET, OP_KRON };
+enum ProfKind { PK_FLIP = 0, PK_GEMM = 1, PK_QR = 2, PK_SOLVE = 3, PK_KRON = 4 };
 
 struct Op {
   int kind;
@@ -103,6 +105,10 @@ struct Chain {
   uint8_t* tr_acc = nullptr;
   double *expk = nullptr, *expki = nullptr;
   float *expk32 = nullptr, *expki32 = nullptr;
+  // E_K = Ex (x) Ey and E_K^-1 = Exi (x) Eyi when the lattice factors (kn1 > 0): the four
+  // 16 x 16 blocks Ex, Ey, Exi, Eyi at kr, zero-padded
+  int kn1 = 0, kn2 = 0;
+  double* kr = nullptr;
   double *Wa[2] = {}, *Wb[2] = {};
   double *LU[2] = {}, *LD[2] = {}, *LV[2] = {};
   double *RU[2] = {}, *RD[2] = {}, *RV[2][2] = {};
@@ -202,6 +208,7 @@ static void layout_chain(const lqmc_ctx* x, Chain& c, char* base, size_t& off) {
     carve(base, off, c.expk32, n2);
     carve(base, off, c.expki32, n2);
   }
+  carve(base, off, c.kr, 4 * 256);
   carve(base, off, c.eye, n2);
   carve(base, off, c.ones, N);
   if (x->opt.delay > 1) {
@@ -258,8 +265,57 @@ struct Builder {
 
   // One launch per kernel class (transpose case, full tiles) present among the tasks; the
   // tasks of one call are independent, so the split launches may run in any order.
+  // An FP64 product with a chain's E_K or E_K^-1 as the untransposed left (right) operand,
+  // with alpha 1 and beta 0, becomes a left (right) Kronecker task when that chain's E_K factors.
+  bool as_kron(const GemmTask& t, int fp32, KronTask& k, int& right) const {
+    if (fp32 || t.alpha != 1.0 || t.beta != 0.0 || t.M != t.N || t.N != t.K) return false;
+    for (const Chain& c : x->ch) {
+      if (c.kn1 == 0 || c.N != t.M) continue;
+      const bool la = !t.transA && (t.A == c.expk || t.A == c.expki) && !t.transB;
+      const bool rb = !t.transB && (t.B == c.expk || t.B == c.expki) && !t.transA;
+      if (!la && !rb) continue;
+      right = la ? 0 : 1;
+      const bool inv = la ? t.A == c.expki : t.B == c.expki;
+      k = KronTask{};
+      k.X = static_cast<const double*>(la ? t.B : t.A);
+      k.C = static_cast<double*>(t.C);
+      k.Ex = c.kr + (inv ? 512 : 0);
+      k.Ey = c.kr + (inv ? 768 : 256);
+      k.rs = t.rs; k.ks = t.ks; k.cs = t.cs;
+      k.rs_kind = t.rs_kind; k.ks_kind = t.ks_kind; k.cs_kind = t.cs_kind;
+      k.N = c.N; k.n1 = c.kn1; k.n2 = c.kn2;
+      k.ldx = la ? t.ldb : t.lda;
+      k.ldc = t.ldc;
+      return true;
+    }
+    return false;
+  }
+
   void gemm(std::vector<Op>& ops, std::vector<GemmTask>& tasks, int fp32) {
     if (tasks.empty()) return;
+    {
+      std::vector<KronTask> kt[2];
+      std::vector<GemmTask> rest;
+      for (auto& t : tasks) {
+        KronTask k;
+        int right = 0;
+        if (as_kron(t, fp32, k, right)) kt[right].push_back(k);
+        else rest.push_back(t);
+      }
+      for (int right = 0; right < 2; ++right) {
+        if (kt[right].empty()) continue;
+        int base = 0;
+        double flops = 0;
+        for (auto& k : kt[right]) {
+          k.tile_base = base;
+          base += kron_tiles(k.N);
+          flops += 2.0 * k.N * (double)k.N * (k.n1 + k.n2);
+        }
+        ops.push_back(Op{OP_KRON, 0, push(kt[right]), (int)kt[right].size(), base, right, 0, flops});
+      }
+      tasks.swap(rest);
+      if (tasks.empty()) return;
+    }
     for (int cls = 0; cls < 8; ++cls) {
       std::vector<GemmTask> part;
       for (auto& t : tasks)
@@ -634,6 +690,7 @@ static void run_ops(lqmc_ctx* x, const std::vector<Op>& ops, int measure, hipStr
     else if (op.kind == OP_GEMM) pk = PK_GEMM;
     else if (op.kind == OP_QR) pk = PK_QR;
     else if (op.kind == OP_SOLVE || op.kind == OP_TRSM) pk = PK_SOLVE;
+    else if (op.kind == OP_KRON) pk = PK_KRON;
     const bool rec = profile && pk == x->prof.kind;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (rec) {
@@ -644,6 +701,7 @@ static void run_ops(lqmc_ctx* x, const std::vector<Op>& ops, int measure, hipStr
     const char* d = x->d_desc + op.off;
     switch (op.kind) {
       case OP_GEMM: launch_gemm(op.fp32, op.nmax, reinterpret_cast<const GemmTask*>(d), op.count, op.ntiles, s); break;
+      case OP_KRON: launch_kron(op.nmax, reinterpret_cast<const KronTask*>(d), op.count, op.ntiles, s); break;
       case OP_QR:
         launch_qr_blocked(reinterpret_cast<const QrTask*>(d), op.count, op.nmax, x->d_state, x->d_qrpub, x->cus, s);
         break;
@@ -865,6 +923,32 @@ static void refresh_chain_desc(lqmc_ctx* x, int ci) {
   x->h_chains[ci] = d;
 }
 
+// E (N x N) = Ex (x) Ey, Ex n1 x n1, Ey n2 x n2, N = n1 n2, site i = a n2 + b?  The factors are read
+// off the first rows of blocks, Ex[a][a'] = E[a n2][a' n2] and Ey[b][b'] = E[b][b'] / E[0][0] (so
+// that Ex[a][a'] Ey[b][b'] = E[a n2][a' n2] E[b][b'] / E[0][0]), and accepted when every entry of
+// E matches its Kronecker product to 1e-13 of max|E|: an E_K = expm(-dtau K) of a hopping matrix
+// K = Kx (x) I + I (x) Ky does (the two terms commute), any other fails.  f: [Ex | Ey] as 16 x 16
+// row-major blocks, zero outside.
+static bool kron_factor(const double* E, int N, int n2, double* f) {
+  const int n1 = N / n2;
+  const double e00 = E[0];
+  if (!(e00 > 0.0)) return false;
+  double emax = 0.0;
+  for (size_t e = 0; e < (size_t)N * N; ++e) emax = std::max(emax, std::fabs(E[e]));
+  std::fill(f, f + 512, 0.0);
+  for (int a = 0; a < n1; ++a)
+    for (int a2 = 0; a2 < n1; ++a2) f[a * 16 + a2] = E[(size_t)a * n2 * N + (size_t)a2 * n2];
+  for (int b = 0; b < n2; ++b)
+    for (int b2 = 0; b2 < n2; ++b2) f[256 + b * 16 + b2] = E[(size_t)b * N + b2] / e00;
+  const double tol = 1e-13 * emax;
+  for (int i = 0; i < N; ++i)
+    for (int k = 0; k < N; ++k) {
+      const double p = f[(i / n2) * 16 + k / n2] * f[256 + (i % n2) * 16 + k % n2];
+      if (!(std::fabs(E[(size_t)i * N + k] - p) <= tol)) return false;
+    }
+  return true;
+}
+
 int lqmc_set_chain(lqmc_ctx* x, int ci, const double* expk, const double* expk_inv, const double* kmat,
                    double lambda, const int8_t* field, uint64_t seed) {
   if (!x || ci < 0 || ci >= x->nchains || !expk || !expk_inv || !kmat || !field)
@@ -891,6 +975,23 @@ int lqmc_set_chain(lqmc_ctx* x, int ci, const double* expk, const double* expk_i
   HIPCHK(hipMemcpy(c.hop, hop.data(), n2 * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c.eye, eye.data(), n2 * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c.ones, ones.data(), N * 8, hipMemcpyHostToDevice));
+  // Kronecker factors of E_K and E_K^-1 (the first split N = n1 n2, 2 <= n1, n2 <= 16, largest
+  // n2 first, that both admit), unless dense products are asked for
+  c.kn1 = c.kn2 = 0;
+  if (!x->opt.dense_ek && x->opt.mode != LQMC_MODE_DET) {
+    std::vector<double> kf(1024);
+    for (int n2c = 16; n2c >= 2; --n2c) {
+      if ((int)N % n2c != 0) continue;
+      const int n1c = (int)N / n2c;
+      if (n1c < 2 || n1c > 16) continue;
+      if (kron_factor(expk, (int)N, n2c, kf.data()) && kron_factor(expk_inv, (int)N, n2c, kf.data() + 512)) {
+        c.kn1 = n1c;
+        c.kn2 = n2c;
+        break;
+      }
+    }
+    if (c.kn1) HIPCHK(hipMemcpy(c.kr, kf.data(), 1024 * 8, hipMemcpyHostToDevice));
+  }
   if (x->opt.precision != LQMC_FP64) {
     std::vector<float> a(n2), b(n2);
     for (size_t e = 0; e < n2; ++e) { a[e] = (float)expk[e]; b[e] = (float)expk_inv[e]; }
@@ -901,6 +1002,13 @@ int lqmc_set_chain(lqmc_ctx* x, int ci, const double* expk, const double* expk_i
   refresh_chain_desc(x, ci);
   HIPCHK(hipMemcpy(x->d_chains + ci, &x->h_chains[ci], sizeof(ChainDev), hipMemcpyHostToDevice));
   x->inited = false;
+  return LQMC_OK;
+}
+
+int lqmc_get_ek_split(lqmc_ctx* x, int ci, int32_t* n1, int32_t* n2) {
+  if (!x || ci < 0 || ci >= x->nchains || !n1 || !n2) return fail(LQMC_EINVAL, "lqmc_get_ek_split: bad arguments");
+  *n1 = x->ch[ci].kn1;
+  *n2 = x->ch[ci].kn2;
   return LQMC_OK;
 }
 
@@ -1208,7 +1316,7 @@ int lqmc_get_observables(lqmc_ctx* x, int ci, double* out, int max_rows, int* ro
 }
 
 int lqmc_profile(lqmc_ctx* x, int kind) {
-  if (!x || kind < -1 || kind > 3) return fail(LQMC_EINVAL, "lqmc_profile: bad kind");
+  if (!x || kind < -1 || kind > 4) return fail(LQMC_EINVAL, "lqmc_profile: bad kind");
   HIPCHK(hipStreamSynchronize(x->stream));
   for (auto& p : x->prof.ev) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
   x->prof.ev.clear();

@@ -31,18 +31,6 @@ template <> struct Mfma<float> {
   __device__ static inline int row(int lane, int r) { return ((lane >> 4) << 2) + r; }
 };
 
-__device__ inline double scale_val(int kind, const double* s, int idx) {
-  if (kind == SC_NONE) return 1.0;
-  double v = s[idx];
-  switch (kind) {
-    case SC_PLAIN: return v;
-    case SC_INV: return 1.0 / v;
-    case SC_INV_BIG: return 1.0 / fmax(v, 1.0);
-    case SC_SMALL: return fmin(v, 1.0);
-    default: return 1.0;
-  }
-}
-
 constexpr int BM = 64, BN = 64, BK = 16, PAD = 4;
 constexpr int GEMM_KMAX = 1024;  // longest inner dimension (the chains' N <= 1024)
 

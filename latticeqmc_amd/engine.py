@@ -71,11 +71,14 @@ class SweepEngine:
               sweeps (``green_tau``); adds two GEMMs and a triangular solve per point
     max_dg    Mode D: a recompute deviating from the propagated G by more than this raises
               a device flag, so ``sync`` fails (0: 1e-3; negative: off)
+    dense_ek  run every product with E_K as a dense GEMM; by default a chain whose E_K is
+              exactly Ex (x) Ey (square and cubic lattices, n1, n2 <= 16) applies the factors
+              in its FP64 products (``ek_split``)
     """
 
     def __init__(self, chains, mode="dqmc", precision="fp64", delay=1, n_stab=10, rng="replay",
                  trace=False, device=0, obs_capacity=4096, flip_threads=0, flip_groups=0, graph=False,
-                 time_displaced=False, max_dg=0.0):
+                 time_displaced=False, max_dg=0.0, dense_ek=False):
         self.lib = nat.load()
         self.chains = list(chains)
         if not self.chains:
@@ -84,7 +87,7 @@ class SweepEngine:
         self.delay, self.n_stab, self.trace_on = int(delay), int(n_stab), bool(trace)
         opt = nat.Options(_MODES[mode], _PREC[precision], int(delay), int(n_stab), _RNG[rng],
                           int(bool(trace)), int(obs_capacity), int(flip_threads), int(flip_groups),
-                          int(bool(time_displaced)), float(max_dg))
+                          int(bool(time_displaced)), float(max_dg), int(bool(dense_ek)))
         ns = (ctypes.c_int32 * len(self.chains))(*[c.n_sites for c in self.chains])
         ls = (ctypes.c_int32 * len(self.chains))(*[c.time_steps for c in self.chains])
         h = ctypes.c_void_p()
@@ -202,8 +205,15 @@ class SweepEngine:
                   "lqmc_get_observables")
         return out[: rows.value]
 
+    def ek_split(self, i):
+        """(n1, n2) of chain i's Kronecker-factored E_K products, or (0, 0) where they are dense."""
+        n1, n2 = ctypes.c_int32(), ctypes.c_int32()
+        nat.check(self.lib.lqmc_get_ek_split(self.h, int(i), ctypes.byref(n1), ctypes.byref(n2)), "lqmc_get_ek_split")
+        return n1.value, n2.value
+
     def profile(self, kind):
-        nat.check(self.lib.lqmc_profile(self.h, {"off": -1, "flip": 0, "gemm": 1, "qr": 2, "solve": 3}[kind]),
+        nat.check(self.lib.lqmc_profile(self.h, {"off": -1, "flip": 0, "gemm": 1, "qr": 2, "solve": 3,
+                                                 "kron": 4}[kind]),
                   "lqmc_profile")
 
     def profile_result(self):

@@ -126,6 +126,47 @@ def test_blocked_qr_race_free(qr_harness_tsan, args):
     assert "WARNING: ThreadSanitizer" not in r.stderr, r.stderr[:4000]
 
 
+def _build_kron(tmp_path_factory, sanitizer):
+    if not os.path.exists(CLANG):
+        pytest.skip("host clang++ with sanitizers not available")
+    out = str(tmp_path_factory.mktemp("emu") / f"kron_harness_{sanitizer}")
+    cmd = [CLANG, "-std=c++20", "-O1", "-g", f"-fsanitize={sanitizer}", "-fno-omit-frame-pointer",
+           "-I", os.path.join(ROOT, "include"), os.path.join(HERE, "cpu_emu", "kron_harness.cpp"),
+           "-o", out, "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    return out
+
+
+@pytest.fixture(scope="module")
+def kron_harness(tmp_path_factory):
+    return _build_kron(tmp_path_factory, "address")
+
+
+@pytest.fixture(scope="module")
+def kron_harness_tsan(tmp_path_factory):
+    return _build_kron(tmp_path_factory, "thread")
+
+
+@pytest.mark.parametrize("args", ["16 16 6 6 1", "3 9 4 16 2", "2 2 16 5 3", "5 7 16 3 4"])
+def test_kron_products(kron_harness, args):
+    """Left and right products with a Kronecker-factored E = Ex (x) Ey (two tasks of different
+    sizes in one launch, ld larger than N in the second, every scale kind) against the dense
+    product to 1e-13 relative, the padding columns of C untouched, every access in bounds and
+    no read of unwritten LDS (it starts as NaN)."""
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
+    r = subprocess.run([kron_harness, *args.split()], capture_output=True, text=True, env=env, timeout=600)
+    assert r.returncode == 0 and "ok:" in r.stdout, (r.stdout + r.stderr)[-3000:]
+
+
+def test_kron_products_race_free(kron_harness_tsan):
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=0 report_signal_unsafe=0")
+    r = subprocess.run([kron_harness_tsan, "6", "6", "3", "9", "5"], capture_output=True, text=True, env=env,
+                       timeout=900)
+    assert "ok:" in r.stdout, (r.stdout + r.stderr)[-3000:]
+    assert "WARNING: ThreadSanitizer" not in r.stderr, r.stderr[:4000]
+
+
 def _build_det(tmp_path_factory, sanitizer):
     if not os.path.exists(CLANG):
         pytest.skip("host clang++ with sanitizers not available")
