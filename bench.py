@@ -109,8 +109,10 @@ def time_form(specs, steps, warmup, profile_kind=None, **kw):
         eng.sync()
         prof = eng.profile_result()
         eng.profile("off")
+    splits = [eng.ek_split(i) for i in range(len(specs))]
     eng.close()
-    return dict(ms_per_step=1e3 * dt / steps, ups=upd * steps / dt, stats=st, prof=prof, upd_per_step=upd, rec=rec)
+    return dict(ms_per_step=1e3 * dt / steps, ups=upd * steps / dt, stats=st, prof=prof, upd_per_step=upd, rec=rec,
+                splits=splits)
 
 
 def cpu_model():
@@ -186,34 +188,50 @@ def _cpu_worker(args):
     return slices * side * side
 
 
-def step_roofline(specs, n_stab, delay, esz, acc, ms_per_step):
+def step_roofline(specs, n_stab, delay, esz, acc, ms_per_step, splits=None):
     """Whole-sweep roofline of the stabilised delayed form: T_roof = F64/P64 + F32/P32 +
     bytes/BW with the algorithmic work per chain and spin
-      left stack   (L - n_s) slice GEMMs 2N^3 + (nb - 1) (QR 8/3 N^3 incl. Q, V update 2N^3)
-      wraps        L x 2 GEMMs (4N^3)            -- in the form's precision
-      right chain  L GEMMs 2N^3 + nb (QR 8/3 N^3 + V update 2N^3)
+      left stack   (L - n_s) slice products E_K X + (nb - 1) (QR 8/3 N^3 incl. Q, V update 2N^3)
+      wraps        L x 2 products with E_K^-1 and E_K   -- in the form's precision
+      right chain  L products E_K X + nb (QR 8/3 N^3 + V update 2N^3)
       recompute    nb x (X1 2N^3 + X2 2N^3 + QR(X) 8/3 N^3 + Z 2N^3 + solve N^3 + G 2N^3)
       flips        L x 2aN^3 (the rank-k products)        -- in the form's precision
     and the flips' G traffic, 4 N^2 s / k bytes per attempted update (both spins' G read and
-    written once per block of k sites).  Diagonal scalings, ratios and copies are not counted."""
-    f64 = f_form = byts = 0.0
-    for sp in specs:
+    written once per block of k sites).  An E_K product is a dense 2N^3 GEMM, or, for a chain
+    whose E_K factors as Ex (x) Ey (splits[i] = (n1, n2); FP64 products), 2 N^2 (n1 + n2) flops
+    and 2 N^2 s bytes (operand read, product written), priced at the larger of flops/P64 and
+    bytes/BW.  Diagonal scalings, ratios and copies are not counted."""
+    f64 = f_form = byts = t_ek = ek_bytes = 0.0
+    for i, sp in enumerate(specs):
         n, L = float(sp.n_sites), sp.time_steps
+        n1, n2 = splits[i] if splits else (0, 0)
         n3 = n ** 3
         nb = -(-L // n_stab)
-        stab = ((L - n_stab) * 2 + (nb - 1) * (8 / 3 + 2) + L * 2 + nb * (8 / 3 + 2)
-                + nb * (2 + 2 + 8 / 3 + 2 + 1 + 2)) * n3
-        form = (L * 4 + L * 2 * acc) * n3
+        stab = ((nb - 1) * (8 / 3 + 2) + nb * (8 / 3 + 2) + nb * (2 + 2 + 8 / 3 + 2 + 1 + 2)) * n3
+        ek64 = (L - n_stab) + L          # E_K products of the stack and the right chain (FP64)
+        ekf = 2 * L                      # the wraps' two products (form precision)
+        form = L * 2 * acc * n3
+        if n1:
+            nk = ek64 + (ekf if esz == 8 else 0)
+            if esz != 8:
+                form += ekf * 2 * n3
+            per_f, per_b = 2 * n * n * (n1 + n2), 2 * n * n * 8
+            t_ek += 2 * nk * max(per_f / (PEAK["fp64"] * 1e12), per_b / (PEAK["hbm"] * 1e9))
+            ek_bytes += 2 * nk * per_b
+        else:
+            stab += ek64 * 2 * n3
+            form += ekf * 2 * n3
         f64 += 2 * stab
         f_form += 2 * form
         byts += 4 * n * n * esz / max(delay, 1) * n * L if delay > 1 else 4 * acc * n * n * esz * n * L
     p_form = PEAK["fp32"] if esz == 4 else PEAK["fp64"]
     if esz == 8:
         f64, f_form = f64 + f_form, 0.0
-    t = (f64 / (PEAK["fp64"] * 1e12) + f_form / (p_form * 1e12) + byts / (PEAK["hbm"] * 1e9)) * 1e3
+    t = (f64 / (PEAK["fp64"] * 1e12) + f_form / (p_form * 1e12) + byts / (PEAK["hbm"] * 1e9)) * 1e3 + t_ek * 1e3
     return {"t_roof_ms": t, "ms_per_step": ms_per_step, "frac": t / ms_per_step, "flops_fp64": f64,
-            "flops_form": f_form, "flip_bytes": byts,
-            "model": "T = F64/78.6 TF + F_form/P_form + flip bytes/8 TB/s, algorithmic counts per sweep"}
+            "flops_form": f_form, "flip_bytes": byts, "ek_factored_bytes": ek_bytes, "ek_factored_ms": t_ek * 1e3,
+            "model": "T = F64/78.6 TF + F_form/P_form + flip bytes/8 TB/s + factored E_K products "
+                     "(max of flops/78.6 TF and bytes/8 TB/s each), algorithmic counts per sweep"}
 
 
 def log(msg):
@@ -239,7 +257,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile-kind", default="flip", help="kernel kind timed with HIP events: flip|gemm|qr|solve")
+    ap.add_argument("--profile-kind", default="flip",
+                    help="kernel kind timed with HIP events: flip|gemm|qr|solve|kron")
     ap.add_argument("--traffic", default=None,
                     help="profiles/ summary (JSON) holding the measured HBM bytes per launch of the profiled "
                          "kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)")
@@ -329,7 +348,9 @@ def main():
     }
     esz = 8 if args.precision == "fp64" else 4
     acc = st["accepted"] / max(st["attempted"], 1)
-    out["step_roofline"] = step_roofline(specs, args.n_stab, args.delay, esz, acc, chosen["ms_per_step"])
+    out["config"]["ek_split"] = chosen["splits"][0] if len(set(map(tuple, chosen["splits"]))) == 1 else "mixed"
+    out["step_roofline"] = step_roofline(specs, args.n_stab, args.delay, esz, acc, chosen["ms_per_step"],
+                                         chosen["splits"])
     if plain:
         out["plain_fp64_value"] = plain["ups"]
         out["plain_ms_per_step"] = plain["ms_per_step"]
@@ -340,7 +361,8 @@ def main():
             pargs.delay, pargs.precision = 1, "fp64"
             out["plain_roofline"] = roofline("flip", *plain["prof"], specs, pargs, plain)
         out["plain_step_roofline"] = step_roofline(specs, args.n_stab, 1, 8, plain["stats"]["accepted"] /
-                                                   max(plain["stats"]["attempted"], 1), plain["ms_per_step"])
+                                                   max(plain["stats"]["attempted"], 1), plain["ms_per_step"],
+                                                   plain["splits"])
     if fp32:
         out["fp32_form"] = {"value": fp32["ups"], "max_dG_vs_fp64": fp32["stats"]["max_err"],
                             "gate_1e-6": bool(fp32["stats"]["max_err"] <= 1e-6)}
@@ -372,6 +394,9 @@ def roofline(kind, avg_ms, launches, work, specs, args, form):
     (78.6 TFLOP/s / 8 TB/s ~ 10), so the bound is memory.  The flop rate is reported
     beside it.
     flip, rank-1 form: 4 a N^2 s bytes per attempt (G streamed per accepted flip).
+    kron: the factored E_K products, memory-bound: each N x N product reads its operand and
+    writes its result once, 2 N^2 * 8 bytes; the engine records 2 N^2 (n1 + n2) flops per
+    product, so the bytes are flops * 8 / (n1 + n2).
     gemm/qr/solve: the flops recorded by the engine per launch, against the FP64 peak.
     """
     st = form["stats"]
@@ -397,6 +422,15 @@ def roofline(kind, avg_ms, launches, work, specs, args, form):
                 "frac": achieved / PEAK["hbm"], "traffic": None, "avg_ms": avg_ms, "launches": launches,
                 "bytes_per_launch": byts, "per_unit": "4*a*N^2*s bytes per attempted flip"}
     flops = work / max(launches, 1)
+    if kind == "kron":
+        n1, n2 = form["splits"][0]
+        byts = flops * 8.0 / max(n1 + n2, 1)
+        achieved = byts / (avg_ms * 1e-3) / 1e9
+        return {"bound": "hbm", "kernel": "k_kron (E_K products through Ex (x) Ey)", "achieved": achieved,
+                "peak": PEAK["hbm"], "unit": "GB/s", "frac": achieved / PEAK["hbm"], "traffic": None,
+                "avg_ms": avg_ms, "launches": launches, "bytes_per_launch": byts,
+                "per_unit": "2*N^2*8 bytes per product (operand read, result written once)",
+                "flops_per_launch": flops, "tflops": flops / (avg_ms * 1e-3) / 1e12}
     achieved = flops / (avg_ms * 1e-3) / 1e12
     peak = peak_mfma if kind == "gemm" else PEAK["fp64"]
     return {"bound": "mfma", "kernel": kind, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",

@@ -28,7 +28,7 @@
 
 namespace lqmc {
 void launch_gemm(int fp32, int cls, const GemmTask* d_tasks, int ntasks, int ntiles, hipStream_t s);
-void launch_kron(int right, const KronTask* d_tasks, int ntasks, int ntiles, hipStream_t s);
+void launch_kron(int right, const KronTask* d_tasks, int ntasks, int ntiles, int cus, hipStream_t s);
 int kron_tiles(int N);
 int gemm_class(const GemmTask& t);
 int gemm_tiles(int M, int N, int* tiles_n);
@@ -701,7 +701,7 @@ static void run_ops(lqmc_ctx* x, const std::vector<Op>& ops, int measure, hipStr
     const char* d = x->d_desc + op.off;
     switch (op.kind) {
       case OP_GEMM: launch_gemm(op.fp32, op.nmax, reinterpret_cast<const GemmTask*>(d), op.count, op.ntiles, s); break;
-      case OP_KRON: launch_kron(op.nmax, reinterpret_cast<const KronTask*>(d), op.count, op.ntiles, s); break;
+      case OP_KRON: launch_kron(op.nmax, reinterpret_cast<const KronTask*>(d), op.count, op.ntiles, x->cus, s); break;
       case OP_QR:
         launch_qr_blocked(reinterpret_cast<const QrTask*>(d), op.count, op.nmax, x->d_state, x->d_qrpub, x->cus, s);
         break;

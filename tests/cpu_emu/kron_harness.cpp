@@ -58,8 +58,10 @@ int main(int argc, char** argv) {
       k.tile_base = base;
       base += kron_tiles(c.N);
     }
-    if (right) emu_launch(base, 256, kron_shmem(), [&] { k_kron<true>(tasks.data(), 2); });
-    else emu_launch(base, 256, kron_shmem(), [&] { k_kron<false>(tasks.data(), 2); });
+    // a grid smaller than the tile count: workgroups walk several tiles, across the two tasks
+    const int grid = kron_grid(base, (base + 11) / 12);
+    if (right) emu_launch(grid, 256, kron_shmem(), [&] { k_kron<true>(tasks.data(), 2, base); });
+    else emu_launch(grid, 256, kron_shmem(), [&] { k_kron<false>(tasks.data(), 2, base); });
     for (int t = 0; t < 2; ++t) {
       Case& c = cs[t];
       const KronTask& k = tasks[t];
