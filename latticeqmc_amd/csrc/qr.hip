@@ -146,24 +146,43 @@ __device__ inline double qr_dpp(double v) {
 // summed over its 16-lane row by four DPP steps (swap within pairs, within quads, rotate
 // the row by 4, by 8).  Reading both halves into values before the conditional keeps v
 // in registers (a conditional over the elements would select their addresses).
+// The two lane-half exchanges as gfx950 permlane swaps (VALU, no LDS round trip): after
+// qr_swap<32>(a, b), lanes 0..31 of a + b hold a summed over lanes l and l + 32 and lanes
+// 32..63 hold b summed likewise; qr_swap<16> does the same for 16-lane rows (0 with 1, 2 with 3).
+template <int HALF>
+__device__ inline void qr_swap(double& a, double& b) {
+  const unsigned long long ua = (unsigned long long)__double_as_longlong(a);
+  const unsigned long long ub = (unsigned long long)__double_as_longlong(b);
+  unsigned alo = (unsigned)ua, ahi = (unsigned)(ua >> 32), blo = (unsigned)ub, bhi = (unsigned)(ub >> 32);
+  if constexpr (HALF == 32) {
+    const auto l = __builtin_amdgcn_permlane32_swap(alo, blo, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false);
+    alo = l[0]; blo = l[1]; ahi = h[0]; bhi = h[1];
+  } else {
+    const auto l = __builtin_amdgcn_permlane16_swap(alo, blo, false, false);
+    const auto h = __builtin_amdgcn_permlane16_swap(ahi, bhi, false, false);
+    alo = l[0]; blo = l[1]; ahi = h[0]; bhi = h[1];
+  }
+  a = __longlong_as_double((long long)(((unsigned long long)ahi << 32) | alo));
+  b = __longlong_as_double((long long)(((unsigned long long)bhi << 32) | blo));
+}
+
 template <int NB>
 __device__ inline void qr_wave_colsum(double (&v)[NB], double* red, int lane) {
   static_assert(NB == 8 || NB == 16, "panel width");
-  {
-    const bool up = (lane & 32) != 0;
+  // column i and i + NB/2 halve over lanes l, l + 32 (then i, i + NB/4 over rows of 16 lanes):
+  // the same pairs of operands as an exchange with lane l ^ 32 (l ^ 16), so the same sums
 #pragma unroll
-    for (int i = 0; i < NB / 2; ++i) {
-      const double lo = v[i], hi = v[NB / 2 + i];
-      v[i] = (up ? hi : lo) + __shfl_xor(up ? lo : hi, 32, 64);
-    }
+  for (int i = 0; i < NB / 2; ++i) {
+    double a = v[i], b = v[NB / 2 + i];
+    qr_swap<32>(a, b);
+    v[i] = a + b;
   }
-  {
-    const bool up = (lane & 16) != 0;
 #pragma unroll
-    for (int i = 0; i < NB / 4; ++i) {
-      const double lo = v[i], hi = v[NB / 4 + i];
-      v[i] = (up ? hi : lo) + __shfl_xor(up ? lo : hi, 16, 64);
-    }
+  for (int i = 0; i < NB / 4; ++i) {
+    double a = v[i], b = v[NB / 4 + i];
+    qr_swap<16>(a, b);
+    v[i] = a + b;
   }
 #pragma unroll
   for (int i = 0; i < NB / 4; ++i) {

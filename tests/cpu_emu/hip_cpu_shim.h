@@ -143,6 +143,22 @@ inline double __shfl(double v, int src, int width = 64) {
   return r;
 }
 
+// gfx950 lane-half swaps, returning {x', y'}: permlane32 swaps lanes 32..63 of x with lanes 0..31
+// of y; permlane16 swaps the odd 16-lane rows of x with the even rows of y.
+struct EmuU2 {
+  unsigned v[2];
+  unsigned operator[](int i) const { return v[i]; }
+};
+inline EmuU2 emu_permlane_swap(unsigned x, unsigned y, int half) {
+  const int lane = threadIdx.x & 63;
+  const unsigned px = (unsigned)__shfl((double)x, lane ^ half);
+  const unsigned py = (unsigned)__shfl((double)y, lane ^ half);
+  const bool upper = (lane & half) != 0;
+  return EmuU2{{upper ? py : x, upper ? y : px}};
+}
+#define __builtin_amdgcn_permlane32_swap(x, y, fi, bc) emu_permlane_swap((x), (y), 32)
+#define __builtin_amdgcn_permlane16_swap(x, y, fi, bc) emu_permlane_swap((x), (y), 16)
+
 // DPP lane exchange within 16-lane rows for the controls the kernels use: quad_perm
 // (ctrl < 0x100) and row_ror:n (0x121..0x12F).
 inline int emu_mov_dpp(int v, int ctrl) {
