@@ -415,9 +415,13 @@ __device__ void form_z(const T* G, size_t n2, int N, int Np, int b0, int kb, int
 // formed from G0[strip rows, B], or from the 16 x 16 copy pin of it when given (a strip
 // whose column groups are shared by several wavefronts: one of them may already have
 // updated the tile holding columns B).
-template <typename T, int K, int RU, bool BF3 = false>
+// FULL: N is a multiple of 16 RU (every strip row and column-tile group lies inside G) and
+// K = 16 (one tile of accepts, m >= 1), so the tile loads and stores of the column loop need no
+// bounds checks; the products are the same.
+template <typename T, int K, int RU, bool BF3 = false, bool FULL = false>
 __device__ void update_strip(T* G, size_t n2, int N, int Np, int b0, int kb, int m, const double* Mx,
                              const T* zws, int rs, int cg = 0, int ng = 1, const double* pin = nullptr) {
+  static_assert(!FULL || K == 16, "full-tile form: one tile of accepts");
   using acc_t = typename MfmaOp<T>::acc_t;
   constexpr int JT = (K + 15) / 16;
   const int lane = threadIdx.x & 63;
@@ -457,14 +461,17 @@ __device__ void update_strip(T* G, size_t n2, int N, int Np, int b0, int kb, int
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int row = r0 + MfmaOp<T>::row(lane, v);
-        acc[u][v] = ldg0(Gs, (size_t)row * N + col, ok && row < N);
+        if constexpr (FULL) acc[u][v] = Gs[(size_t)row * N + col];
+        else acc[u][v] = ldg0(Gs, (size_t)row * N + col, ok && row < N);
       }
 #pragma unroll
       for (int jt = 0; jt < JT; ++jt)
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
-          zv[u][jt][v] = ldg0(Zg, ((size_t)((sp * nct + ct) * JT + jt) * 4 + v) * 64 + lane,
-                              ct < nct && jt < jtu);
+        for (int v = 0; v < 4; ++v) {
+          const size_t zi = ((size_t)((sp * nct + ct) * JT + jt) * 4 + v) * 64 + lane;
+          if constexpr (FULL) zv[u][jt][v] = Zg[zi];
+          else zv[u][jt][v] = ldg0(Zg, zi, ct < nct && jt < jtu);
+        }
     }
     if constexpr (BF3) {
       // FP32 operands: lane l's P^T accumulator elements v = 0..3 are P[l % 16][4 (l / 16) + v] and
@@ -500,7 +507,7 @@ __device__ void update_strip(T* G, size_t n2, int N, int Np, int b0, int kb, int
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int row = r0 + MfmaOp<T>::row(lane, v);
-        if (ct < nct && row < N && col < N) Gs[(size_t)row * N + col] = acc[u][v];
+        if (FULL || (ct < nct && row < N && col < N)) Gs[(size_t)row * N + col] = acc[u][v];
       }
     }
   }
@@ -676,6 +683,8 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
   int spmask = 0;
   for (int rs = 0; rs < 2 * nct; ++rs)
     if ((rs % tw) / nw == w) spmask |= 1 << (rs / nct);
+  // whole strips and column-tile groups of 4 tiles: the strip updates need no bounds checks
+  const bool full = K == 16 && N % 64 == 0;
   for (int i = threadIdx.x; i < N; i += blockDim.x) fl[i] = fld[i];
   __syncthreads();
   bool pending = false;
@@ -700,7 +709,8 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
         if ((rs % tw) / nw != w) continue;
         for (int g = 0; g < ngr; ++g, ++item)
           if (wave == w0 + item % nwv)
-            update_strip<T, K, RUd, BF3>(G, n2, N, Np, pb0, pkb, pm, Mxp, wsT, rs, g, ngr, Pin + sp * 256);
+            if (full) update_strip<T, K, RUd, BF3, K == 16>(G, n2, N, Np, pb0, pkb, pm, Mxp, wsT, rs, g, ngr, Pin + sp * 256);
+            else update_strip<T, K, RUd, BF3>(G, n2, N, Np, pb0, pkb, pm, Mxp, wsT, rs, g, ngr, Pin + sp * 256);
       }
     } else {
       int i = 0;
@@ -709,7 +719,7 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
           const int rs = sp * nct + st, own = rs % tw;
           if (own / nw != w) continue;
           const int dw = serial_busy ? 1 + i % (nw - 1) : own % nw;
-          if (wave == dw) update_strip<T, K, RUd, BF3>(G, n2, N, Np, pb0, pkb, pm, Mxp, wsT, rs);
+          if (wave == dw) update_strip<T, K, RUd, BF3>(G, n2, N, Np, pb0, pkb, pm, Mxp, wsT, rs);  // K = 32
         }
     }
   };
@@ -921,7 +931,10 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
       const int s0 = defer ? b0 >> 4 : nct, s1 = defer ? (b0 + kb - 1) >> 4 : -1;
       for (int rs = gw; rs < 2 * nct; rs += tw) {
         const int st = rs % nct;
-        if (st < s0 || st > s1) update_strip<T, K, (K >= 16 ? 4 : 2), BF3>(G, n2, N, Np, b0, kb, m, Xsb, wsT, rs);
+        if (st < s0 || st > s1) {
+          if (full) update_strip<T, K, (K >= 16 ? 4 : 2), BF3, K == 16>(G, n2, N, Np, b0, kb, m, Xsb, wsT, rs);
+          else update_strip<T, K, (K >= 16 ? 4 : 2), BF3>(G, n2, N, Np, b0, kb, m, Xsb, wsT, rs);
+        }
       }
     }
     if (K >= 16 && W > 1) {
