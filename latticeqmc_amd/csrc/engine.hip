@@ -28,7 +28,7 @@
 
 namespace lqmc {
 void launch_gemm(int fp32, int cls, const GemmTask* d_tasks, int ntasks, int ntiles, hipStream_t s);
-void launch_kron(int right, const KronTask* d_tasks, int ntasks, int ntiles, int cus, hipStream_t s);
+void launch_kron(int right, int all16, const KronTask* d_tasks, int ntasks, int ntiles, int cus, hipStream_t s);
 int kron_tiles(int N);
 int gemm_class(const GemmTask& t);
 int gemm_tiles(int M, int N, int* tiles_n);
@@ -304,14 +304,16 @@ struct Builder {
       }
       for (int right = 0; right < 2; ++right) {
         if (kt[right].empty()) continue;
-        int base = 0;
+        int base = 0, all16 = 1;
         double flops = 0;
         for (auto& k : kt[right]) {
           k.tile_base = base;
           base += kron_tiles(k.N);
           flops += 2.0 * k.N * (double)k.N * (k.n1 + k.n2);
+          all16 &= k.n1 == 16 && k.n2 == 16;
         }
-        ops.push_back(Op{OP_KRON, 0, push(kt[right]), (int)kt[right].size(), base, right, 0, flops});
+        // fp32 field: whether every task is 16 x 16 (the kernel's form without bounds checks)
+        ops.push_back(Op{OP_KRON, all16, push(kt[right]), (int)kt[right].size(), base, right, 0, flops});
       }
       tasks.swap(rest);
       if (tasks.empty()) return;
@@ -701,7 +703,7 @@ static void run_ops(lqmc_ctx* x, const std::vector<Op>& ops, int measure, hipStr
     const char* d = x->d_desc + op.off;
     switch (op.kind) {
       case OP_GEMM: launch_gemm(op.fp32, op.nmax, reinterpret_cast<const GemmTask*>(d), op.count, op.ntiles, s); break;
-      case OP_KRON: launch_kron(op.nmax, reinterpret_cast<const KronTask*>(d), op.count, op.ntiles, x->cus, s); break;
+      case OP_KRON: launch_kron(op.nmax, op.fp32, reinterpret_cast<const KronTask*>(d), op.count, op.ntiles, x->cus, s); break;
       case OP_QR:
         launch_qr_blocked(reinterpret_cast<const QrTask*>(d), op.count, op.nmax, x->d_state, x->d_qrpub, x->cus, s);
         break;

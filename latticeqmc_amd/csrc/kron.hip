@@ -57,7 +57,7 @@ __device__ inline double kr_scale_of(int kind, double v) {
 }
 
 // The tile's operand share (v) and entry tid of the task's three scale vectors (raw).
-template <bool RIGHT>
+template <bool RIGHT, bool F16>
 __device__ inline void kr_load(const KronTask& tk, int t0, double (&v)[16], double (&sc)[3]) {
   const int tid = threadIdx.x;
   sc[0] = tk.ks_kind != SC_NONE && tid < tk.N ? tk.ks[tid] : 1.0;
@@ -68,9 +68,11 @@ __device__ inline void kr_load(const KronTask& tk, int t0, double (&v)[16], doub
   for (int u = 0; u < 16; ++u) {
     int ap, bp, j;
     kr_coords<RIGHT>((int)threadIdx.x + 256 * u, ap, bp, j);
-    const bool ok = ap < tk.n1 && bp < tk.n2 && t0 + j < tk.N;
+    const bool ok = F16 || (ap < tk.n1 && bp < tk.n2 && t0 + j < tk.N);
     const int k = ap * tk.n2 + bp;
-    v[u] = ldg0(X, RIGHT ? (size_t)(t0 + j) * tk.ldx + k : (size_t)k * tk.ldx + t0 + j, ok);
+    const size_t ix = RIGHT ? (size_t)(t0 + j) * tk.ldx + k : (size_t)k * tk.ldx + t0 + j;
+    if constexpr (F16) v[u] = X[ix];
+    else v[u] = ldg0(X, ix, ok);
   }
 }
 
@@ -78,7 +80,7 @@ __device__ inline void kr_load(const KronTask& tk, int t0, double (&v)[16], doub
 // applied to the rows of X.  Each workgroup walks tiles blockIdx.x, + gridDim.x, ...; the next
 // tile's operand loads are issued before the current tile's contractions, so they are in flight
 // during them.
-template <bool RIGHT>
+template <bool RIGHT, bool F16>
 __global__ __launch_bounds__(256, 4) void k_kron(const KronTask* __restrict__ tasks, int ntasks, int ntiles) {
   int tile = blockIdx.x;
   if (tile >= ntiles) return;
@@ -90,9 +92,10 @@ __global__ __launch_bounds__(256, 4) void k_kron(const KronTask* __restrict__ ta
   KronTask tk = tasks[kr_task(tasks, ntasks, tile)];
   int t0 = (tile - tk.tile_base) * KR_T;  // first column (left) or row (right) of the tile
   double v[16], sc[3];
-  kr_load<RIGHT>(tk, t0, v, sc);
+  kr_load<RIGHT, F16>(tk, t0, v, sc);
   for (;;) {
-    const int N = tk.N, n1 = tk.n1, n2 = tk.n2;
+    // F16: every task of the launch is 16 x 16 (N = 256): no tile reaches past the lattice
+    const int N = F16 ? 256 : tk.N, n1 = F16 ? 16 : tk.n1, n2 = F16 ? 16 : tk.n2;
     // 1. V[a'][b'][j] = X[k][t0 + j] (left) or X[t0 + j][k] (right), k = a' n2 + b'; zero outside
     //    a' < n1, b' < n2, t0 + j < N.  The scales to LDS (ks is applied to the stage-1 operands).
 #pragma unroll
@@ -121,7 +124,7 @@ __global__ __launch_bounds__(256, 4) void k_kron(const KronTask* __restrict__ ta
     if (more) {
       tn = tasks[kr_task(tasks, ntasks, nt)];
       n0 = (nt - tn.tile_base) * KR_T;
-      kr_load<RIGHT>(tn, n0, v, sc);
+      kr_load<RIGHT, F16>(tn, n0, v, sc);
     }
     __syncthreads();
 
@@ -138,7 +141,7 @@ __global__ __launch_bounds__(256, 4) void k_kron(const KronTask* __restrict__ ta
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
           const int ap = wave + 8 * h + 4 * q, bp = 4 * s + lk;
-          const double kf = ap < n1 && bp < n2 ? ksl[ap * n2 + bp] : 0.0;
+          const double kf = F16 || (ap < n1 && bp < n2) ? ksl[ap * n2 + bp] : 0.0;
           bv[q][s] = vs[ap * KR_AS + bp * KR_BS + li] * kf;
         }
 #pragma unroll
@@ -172,7 +175,7 @@ __global__ __launch_bounds__(256, 4) void k_kron(const KronTask* __restrict__ ta
       if (!RIGHT) {
         // C[a n2 + b][t0 + j]: 16 consecutive columns per row
         const int c = t0 + li;
-        if (c < N) {
+        if (F16 || c < N) {
           const double cv = csl[c];
 #pragma unroll
           for (int q = 0; q < 2; ++q) {
@@ -180,7 +183,7 @@ __global__ __launch_bounds__(256, 4) void k_kron(const KronTask* __restrict__ ta
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               const int a = lk + 4 * r;
-              if (a < n1 && b < n2) {
+              if (F16 || (a < n1 && b < n2)) {
                 const int row = a * n2 + b;
                 tk.C[(size_t)row * tk.ldc + c] = rsl[row] * acc[q][r] * cv;
               }
@@ -203,7 +206,7 @@ __global__ __launch_bounds__(256, 4) void k_kron(const KronTask* __restrict__ ta
         const int e = tid + 256 * u;
         const int j = e >> 8, a = (e >> 4) & 15, b = e & 15;
         const int row = t0 + j, col = a * n2 + b;
-        if (a < n1 && b < n2 && row < N)
+        if (F16 || (a < n1 && b < n2 && row < N))
           tk.C[(size_t)row * tk.ldc + col] =
               rsl[row] * vs[a * KR_AS + b * KR_BS + j] * csl[col];
       }
@@ -222,11 +225,14 @@ size_t kron_shmem() { return sizeof(double) * (16 * KR_AS + 3 * 256); }  // 40 K
 // allows), each walking ntiles / grid tiles.
 int kron_grid(int ntiles, int cus) { return std::max(1, std::min(ntiles, 4 * std::max(cus, 1))); }
 
-void launch_kron(int right, const KronTask* d_tasks, int ntasks, int ntiles, int cus, hipStream_t s) {
+// all16: every task of the launch has n1 = n2 = 16.
+void launch_kron(int right, int all16, const KronTask* d_tasks, int ntasks, int ntiles, int cus, hipStream_t s) {
   if (ntasks <= 0 || ntiles <= 0) return;
   const int grid = kron_grid(ntiles, cus);
-  if (right) hipLaunchKernelGGL(k_kron<true>, dim3(grid), dim3(256), kron_shmem(), s, d_tasks, ntasks, ntiles);
-  else hipLaunchKernelGGL(k_kron<false>, dim3(grid), dim3(256), kron_shmem(), s, d_tasks, ntasks, ntiles);
+  if (right && all16) hipLaunchKernelGGL((k_kron<true, true>), dim3(grid), dim3(256), kron_shmem(), s, d_tasks, ntasks, ntiles);
+  else if (right) hipLaunchKernelGGL((k_kron<true, false>), dim3(grid), dim3(256), kron_shmem(), s, d_tasks, ntasks, ntiles);
+  else if (all16) hipLaunchKernelGGL((k_kron<false, true>), dim3(grid), dim3(256), kron_shmem(), s, d_tasks, ntasks, ntiles);
+  else hipLaunchKernelGGL((k_kron<false, false>), dim3(grid), dim3(256), kron_shmem(), s, d_tasks, ntasks, ntiles);
 }
 
 int kron_tiles(int N) { return (N + KR_T - 1) / KR_T; }

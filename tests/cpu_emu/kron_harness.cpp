@@ -60,8 +60,12 @@ int main(int argc, char** argv) {
     }
     // a grid smaller than the tile count: workgroups walk several tiles, across the two tasks
     const int grid = kron_grid(base, (base + 11) / 12);
-    if (right) emu_launch(grid, 256, kron_shmem(), [&] { k_kron<true>(tasks.data(), 2, base); });
-    else emu_launch(grid, 256, kron_shmem(), [&] { k_kron<false>(tasks.data(), 2, base); });
+    // the form without bounds checks when both tasks are 16 x 16
+    const bool all16 = cs[0].n1 == 16 && cs[0].n2 == 16 && cs[1].n1 == 16 && cs[1].n2 == 16;
+    if (right && all16) emu_launch(grid, 256, kron_shmem(), [&] { k_kron<true, true>(tasks.data(), 2, base); });
+    else if (right) emu_launch(grid, 256, kron_shmem(), [&] { k_kron<true, false>(tasks.data(), 2, base); });
+    else if (all16) emu_launch(grid, 256, kron_shmem(), [&] { k_kron<false, true>(tasks.data(), 2, base); });
+    else emu_launch(grid, 256, kron_shmem(), [&] { k_kron<false, false>(tasks.data(), 2, base); });
     for (int t = 0; t < 2; ++t) {
       Case& c = cs[t];
       const KronTask& k = tasks[t];

@@ -148,7 +148,7 @@ def kron_harness_tsan(tmp_path_factory):
     return _build_kron(tmp_path_factory, "thread")
 
 
-@pytest.mark.parametrize("args", ["16 16 6 6 1", "3 9 4 16 2", "2 2 16 5 3", "5 7 16 3 4"])
+@pytest.mark.parametrize("args", ["16 16 6 6 1", "16 16 16 16 6", "3 9 4 16 2", "2 2 16 5 3", "5 7 16 3 4"])
 def test_kron_products(kron_harness, args):
     """Left and right products with a Kronecker-factored E = Ex (x) Ey (two tasks of different
     sizes in one launch, ld larger than N in the second, every scale kind) against the dense
