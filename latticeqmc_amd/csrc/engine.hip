@@ -44,6 +44,8 @@ void launch_flip(int fp32, int delayed, const FlipArgs& a, int nchains, int thre
                  hipStream_t s);
 size_t flip_delayed_shmem(int nmax, int k, int threads, int fp32);
 size_t flip_delayed_workspace(int n, int k, int fp32, int groups);
+int flip_delayed_stage_rows(int nmax, int k, int groups);
+size_t flip_delayed_rows_shmem(int nmax, int k, int groups);
 size_t flip_rank1_split_shmem(int nmax, int threads, int groups);
 int flip_grid(int nchains, int groups);
 int flip_delayed_threads_limit(int k);
@@ -717,7 +719,8 @@ static void run_ops(lqmc_ctx* x, const std::vector<Op>& ops, int measure, hipStr
         break;
       case OP_FLIP: {
         FlipArgs a{x->d_chains, x->d_state, op.t, measure, x->opt.rng, x->opt.trace, x->opt.delay,
-                   x->flip_groups, x->d_bar, x->nchains, x->opt.precision == LQMC_BF16X3 ? 1 : 0};
+                   x->flip_groups, x->d_bar, x->nchains, x->opt.precision == LQMC_BF16X3 ? 1 : 0,
+                   x->opt.delay > 1 ? flip_delayed_stage_rows(x->nmax, x->opt.delay, x->flip_groups) : 0};
         launch_flip(fp32, x->opt.delay > 1, a, x->nchains, x->flip_threads, x->flip_shmem, s);
         break;
       }
@@ -856,6 +859,12 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
         return fail(LQMC_EINVAL, "split rank-1 flip needs more than 64 KB of LDS");
       }
     }
+  }
+  // rows of each block staged in LDS for the Z tiles (16 x nmax doubles, nmax <= 256)
+  if (o.delay > 1) x->flip_shmem += flip_delayed_rows_shmem(x->nmax, o.delay, x->flip_groups);
+  if ((int)x->flip_shmem > maxsh) {
+    delete x;
+    return fail(LQMC_EINVAL, "flip kernel needs more than 64 KB of LDS; lower the delay");
   }
   if (o.mode == LQMC_MODE_DET) {
     x->det_shmem = det_sweep_shmem(x->nmax, x->lmax);

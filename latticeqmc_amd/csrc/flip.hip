@@ -361,9 +361,11 @@ __device__ inline void wave_sync() {
 // updates only its own strip.
 // spmask: bit s set = the tiles of spin s are formed (a workgroup forms only the spins of the
 // strips it updates).
-template <typename T, int K>
+// LROWS: the workgroup forms one spin's tiles and rows B of that spin are in LDS (Yl, leading
+// dimension N), read in place of G's rows.
+template <typename T, int K, bool LROWS = false>
 __device__ void form_z(const T* G, size_t n2, int N, int Np, int b0, int kb, int m, const double* My, T* zws,
-                       int spmask = 3) {
+                       int spmask = 3, const T* Yl = nullptr) {
   using acc_t = typename MfmaOp<T>::acc_t;
   constexpr int JT = (K + 15) / 16;  // 16-wide tiles over the accepts
   constexpr int KS = K / 4;           // k-steps of one tile
@@ -391,7 +393,9 @@ __device__ void form_z(const T* G, size_t n2, int N, int Np, int b0, int kb, int
       for (int ks = 0; ks < KS; ++ks) {
         const int q = 4 * ks + lk;
         xv[u][ks] = (T)ld0(Mys, (size_t)q * K + a, ok && q < kb && a < m);
-        T y = -ldg0(Gs, (size_t)(b0 + q) * N + c, ok && q < kb && c < N);
+        T y;
+        if constexpr (LROWS) y = -(ok && q < kb && c < N ? Yl[q * N + c] : T(0));
+        else y = -ldg0(Gs, (size_t)(b0 + q) * N + c, ok && q < kb && c < N);
         if (ok && q < kb && c == b0 + q) y = y + T(1);
         yv[u][ks] = y;
       }
@@ -671,7 +675,8 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
   int* m_sh = sb + K;                          // [4] accept count of the block; [1] barrier flag
   double* Mxp = reinterpret_cast<double*>(m_sh + 4);  // [2][K][K] Mx of the previous block (W > 1)
   double* Pin = Mxp + 2 * K * K;  // [2][16][16] G0[rows, columns] of the deferred strips (K = 16)
-  int8_t* fl = reinterpret_cast<int8_t*>(Pin + (K == 16 ? 512 : 0));  // [N] the slice's field
+  T* Yl = reinterpret_cast<T*>(Pin + (K == 16 ? 512 : 0));  // [16][N] rows B of the workgroup's spin (a.yrows)
+  int8_t* fl = reinterpret_cast<int8_t*>(Yl + (K == 16 && a.yrows ? 16 * N : 0));  // [N] the slice's field
 
   const unsigned long long sweep = a.st[c].sweep;
   unsigned long long nacc = 0;
@@ -685,6 +690,11 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
     if ((rs % tw) / nw == w) spmask |= 1 << (rs / nct);
   // whole strips and column-tile groups of 4 tiles: the strip updates need no bounds checks
   const bool full = K == 16 && N % 64 == 0;
+  // one spin per workgroup (W > 1): rows B of that spin are staged in LDS by the helper waves
+  // (below) while wave 0 runs the serial phase, and the Z tiles read them there.  Rows B are final after
+  // the block's chain barrier and stay untouched until the next one (the strips holding them
+  // are deferred), so the staged rows equal what the Z tiles would read from G.
+  const bool lrows = K == 16 && a.yrows && W > 1 && (spmask == 1 || spmask == 2);
   for (int i = threadIdx.x; i < N; i += blockDim.x) fl[i] = fld[i];
   __syncthreads();
   bool pending = false;
@@ -697,10 +707,15 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
   // left to one wavefront's latency.
   // For K = 16 (one strip per spin) each strip's column-tile groups are shared out over
   // those waves, with P formed from the copy Pin; for K = 32 a wave takes whole strips.
+  // While wave 0 runs the serial phase, the other work of the block goes to the waves that
+  // do not share its SIMD (wave w runs on SIMD w % 4): helper i is wave i + i / 3 + 1, i.e.
+  // waves 1, 2, 3, 5, 6, 7 of eight (nw - (nw + 3) / 4 helpers).
+  auto helper_wave = [](int i) { return i + i / 3 + 1; };
   auto deferred_strips = [&](bool serial_busy) {
     constexpr int RUd = K >= 16 ? 4 : 2;
     const int s0 = pb0 >> 4, s1 = (pb0 + pkb - 1) >> 4;
     const int w0 = serial_busy ? 1 : 0, nwv = nw - w0;
+    const int nhw = nw - (nw + 3) / 4;  // helper waves
     if constexpr (K == 16) {
       const int ngr = (nct + RUd - 1) / RUd;  // column-tile groups per strip
       int item = 0;                           // (strip, group) items of this workgroup
@@ -708,7 +723,7 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
         const int rs = sp * nct + s0;
         if ((rs % tw) / nw != w) continue;
         for (int g = 0; g < ngr; ++g, ++item)
-          if (wave == w0 + item % nwv)
+          if (wave == (serial_busy ? helper_wave(item % nhw) : w0 + item % nwv))
             if (full) update_strip<T, K, RUd, BF3, K == 16>(G, n2, N, Np, pb0, pkb, pm, Mxp, wsT, rs, g, ngr, Pin + sp * 256);
             else update_strip<T, K, RUd, BF3>(G, n2, N, Np, pb0, pkb, pm, Mxp, wsT, rs, g, ngr, Pin + sp * 256);
       }
@@ -829,8 +844,22 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
         }
       }
       if (lane == 0) m_sh[0] = m;
-    } else if (K >= 16 && pending) {
-      deferred_strips(true);
+    } else if (K >= 16) {
+      if (lrows) {
+        const LQMC_GLOBAL T* Gs = gptr(G) + (spmask >> 1) * n2 + (size_t)b0 * N;
+        const int hi = wave - 1 - (wave - 1) / 4;  // helper index of this wave
+        const int nt = 64 * (nw - (nw + 3) / 4), tt = 64 * hi + lane, tot = kb * N;
+        if (wave % 4 != 0)
+          for (int e0 = tt; e0 < tot; e0 += 8 * nt) {
+            T v[8];  // every load of the batch first
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = ldg0(Gs, (size_t)(e0 + u * nt), e0 + u * nt < tot);
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+              if (e0 + u * nt < tot) Yl[e0 + u * nt] = v[u];
+          }
+      }
+      if (pending) deferred_strips(true);
     }
     __syncthreads();
     pending = false;
@@ -923,7 +952,8 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
         if (r <= j) out[aidx[r] * K + j] = lower ? inv[r] : cs[r] * inv[r];
     }
     __syncthreads();
-    form_z<T, K>(G, n2, N, Np, b0, kb, m, Ysb, wsT, spmask);
+    if (lrows) form_z<T, K, true>(G, n2, N, Np, b0, kb, m, Ysb, wsT, spmask, Yl);
+    else form_z<T, K>(G, n2, N, Np, b0, kb, m, Ysb, wsT, spmask);
     __syncthreads();
     {
       // W > 1: the strips of this block's rows wait for the next barrier (see above)
@@ -1334,6 +1364,14 @@ size_t flip_delayed_shmem(int nmax, int k, int threads, int fp32) {
   (void)fp32;
   return sizeof(double) * (threads + 6 * (size_t)k * k + 3 * (size_t)k + (k == 16 ? 512 : 0)) +
          sizeof(int) * (2 * k + 4) + (size_t)nmax;
+}
+
+// Whether the delayed flip stages each block's rows in LDS (FlipArgs::yrows): 16-site blocks
+// on several workgroups per chain, N <= 256 (16 x nmax doubles, at most 32 KB, added to
+// flip_delayed_shmem).
+int flip_delayed_stage_rows(int nmax, int k, int groups) { return k == 16 && groups > 1 && nmax <= 256 ? 1 : 0; }
+size_t flip_delayed_rows_shmem(int nmax, int k, int groups) {
+  return flip_delayed_stage_rows(nmax, k, groups) ? sizeof(double) * 16 * (size_t)nmax : 0;
 }
 
 // Bytes of the per-chain delayed-update workspace (the Z tiles of both spins, one set per

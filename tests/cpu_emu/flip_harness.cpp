@@ -62,12 +62,13 @@ int main(int argc, char** argv) {
   }
   std::memset(st.data(), 0, sizeof(ChainState) * nch);
   std::vector<unsigned> bar(4 * nch, 0u);
-  FlipArgs a{chains.data(), st.data(), t, measure, 1, 1, K, groups, bar.data(), nch, fp32 == 2 ? 1 : 0};
+  FlipArgs a{chains.data(), st.data(), t, measure, 1, 1, K, groups, bar.data(), nch, fp32 == 2 ? 1 : 0,
+              K > 1 ? flip_delayed_stage_rows(N, K, groups) : 0};
   if (K > 1 && (threads > flip_delayed_threads_limit(K) || threads < 256)) {
     std::fprintf(stderr, "threads above the delayed kernel's launch bound %d\n", flip_delayed_threads_limit(K));
     return 2;
   }
-  const size_t lds = K > 1 ? flip_delayed_shmem(N, K, threads, fp32)
+  const size_t lds = K > 1 ? flip_delayed_shmem(N, K, threads, fp32) + flip_delayed_rows_shmem(N, K, groups)
                    : groups > 1 ? flip_rank1_split_shmem(N, threads, groups)
                                 : sizeof(double) * (4 * (size_t)N + threads);
   std::printf("N=%d L=%d K=%d threads=%d fp32=%d chains=%d t=%d groups=%d lds=%zu\n", N, L, K, threads, fp32, nch, t,

@@ -45,6 +45,7 @@ def harness_tsan(tmp_path_factory):
     "100 6 16 256 0 2 1 0 200 2",   # two workgroups per chain, ragged, all blocks concurrent
     "100 6 32 256 0 1 5 1 200 3",   # three workgroups per chain, k = 32, measurement slice
     "256 3 16 512 0 1 2 1 120 4",   # the throughput form: N = 256, 512 threads, four workgroups
+    "64 3 16 256 0 2 2 1 100 4",    # four workgroups of 256 threads, one spin each: rows staged in LDS
     "64 40 8 256 2 2 39 1 30",      # split-BF16 rank-k products (FP32 G), measurement slice
     "64 16 16 512 2 1 15 1 40",     # split-BF16, 16-site blocks on the 512-thread bound
     "100 4 1 256 0 3 3 1 80 3",     # rank-1 on three workgroups per chain, three chains, uneven rows
@@ -68,6 +69,7 @@ def test_flip_kernel_in_bounds(harness, args):
     "64 16 1 256 0 1 0 1",      # rank-1
     "64 3 16 256 0 1 2 1 100 2",  # two workgroups per chain: the chain barrier orders G
     "256 2 16 512 0 1 1 1 100 2",  # the throughput form's N = 256 on 512 threads, two workgroups
+    "64 3 16 256 0 1 2 1 100 4",   # one spin per workgroup: rows B staged in LDS while wave 0 runs the sites
     "64 3 1 256 0 1 2 1 50 2",     # rank-1 on two workgroups: the barrier after each accepted flip
 ])
 def test_flip_kernel_race_free(harness_tsan, args):
