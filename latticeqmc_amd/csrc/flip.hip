@@ -331,6 +331,24 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Lane (l & ~3) + src of each lane quad, for a double: a DPP quad_perm broadcast (a VALU
+// operation, no LDS round trip).  src is 0..3 and a constant once the caller's loop unrolls.
+template <int CTRL>
+__device__ inline double quad_dpp(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xF, 0xF, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ inline double quad_bcast(double v, int src) {
+  switch (src) {
+    case 0: return quad_dpp<0x00>(v);
+    case 1: return quad_dpp<0x55>(v);
+    case 2: return quad_dpp<0xAA>(v);
+    default: return quad_dpp<0xFF>(v);
+  }
+}
+
 // Per block of k sites (k a multiple of 4):
 //  1. the block's field entries and uniforms go to LDS;
 //  2. wave 0 runs the k attempts on the k x k diagonal blocks of both spins, held in
@@ -895,61 +913,99 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
     }
     __syncthreads();
     // unit upper-triangular inverses, one per wave: waves 0/1 spin 0 (I+UC)^-1 / (I-L)^-1,
-    // waves 2/3 spin 1; lane j forms column j by back substitution in registers (the
-    // loop over rows stays rolled; inv is written through a select chain)
-    double inv[K];
+    // waves 2/3 spin 1, then scattered onto the block rows:
+    // Mx[aidx[r]][j] = c_r inv[r][j], My[aidx[r]][j] = inv[r][j] (zero on rejected sites).
     const int isp = wave >> 1;
     const bool lower = (wave & 1) != 0;
-    const int j = lane;
-    if (wave < 4 && K <= 16) {
-      // fully unrolled: row r sums only its entries p > r (the table is strictly upper
-      // triangular with exact zeros elsewhere, so the result is the same)
-      const double* Tw = Xsb + isp * 2 * K * K + (lower ? K * K : 0);
+    if constexpr (K <= 16) {
+      // Lane 4 j + g holds rows r = g + 4 i (i < K / 4) of column j.  Column-oriented back
+      // substitution: once row p of column j is final (on lane 4 j + p % 4) it is broadcast
+      // through the lane quad (DPP) and every row r < p takes its term -T[r][p] inv[p] in one
+      // fused multiply-add.  The table is strictly upper triangular with exact zeros
+      // elsewhere, so rows r >= p take zero terms and need no mask.  Every table entry a
+      // lane uses is read into registers first: the substitution waits on no LDS read.
+      constexpr int NI = K / 4;
+      const int j = lane >> 2, g = lane & 3;
+      double iv[NI];
+      if (wave < 4) {
+        const double* Tw = Xsb + isp * 2 * K * K + (lower ? K * K : 0);
+        double tr[NI][K];  // tr[i][p] = T[g + 4 i][p], for p > 4 i
 #pragma unroll
-      for (int p = 0; p < K; ++p) inv[p] = p == j ? 1.0 : 0.0;
+        for (int i = 0; i < NI; ++i)
 #pragma unroll
-      for (int r = K - 2; r >= 0; --r) {
-        double v4[4] = {0.0, 0.0, 0.0, 0.0};
+          for (int p = 4 * i + 1; p < K; ++p) tr[i][p] = Tw[(g + 4 * i) * K + p];
 #pragma unroll
-        for (int p = r + 1; p < K; ++p) v4[p & 3] -= Tw[r * K + p] * inv[p];
-        const double v = (v4[0] + v4[1]) + (v4[2] + v4[3]);
-        if (r < j) inv[r] = v;
+        for (int i = 0; i < NI; ++i) iv[i] = g + 4 * i == j ? 1.0 : 0.0;
+#pragma unroll
+        for (int p = K - 1; p >= 1; --p) {
+          const double ip = quad_bcast(iv[p / 4], p % 4);
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            if (4 * i < p) iv[i] = fma(-tr[i][p], ip, iv[i]);
+        }
       }
-    } else if (wave < 4) {
-      const double* Tw = Xsb + isp * 2 * K * K + (lower ? K * K : 0);
+      __syncthreads();
+      if (wave < 4) {
+        const bool live = j < m;
+        double* out = (lower ? Ysb : Xsb) + isp * K * K;
+        const double* cs = cf + isp * K;
+        int ar[NI];
+        double cr[NI];
 #pragma unroll
-      for (int p = 0; p < K; ++p) inv[p] = p == j ? 1.0 : 0.0;
-      // row r of the table is read while row r+1 is being used (one LDS wait per row)
-      double trow[K];
+        for (int i = 0; i < NI; ++i) {
+          ar[i] = aidx[g + 4 * i];  // past the accepts: read, not used
+          cr[i] = cs[g + 4 * i];
+        }
+        // rows qq = g + 4 i of column j zeroed first, then the accepted rows stored over them
+        // by (other lanes of) the same wavefront after a wavefront barrier
 #pragma unroll
-      for (int p = 0; p < K; ++p) trow[p] = Tw[(K - 2) * K + p];
+        for (int i = 0; i < NI; ++i)
+          if (live && g + 4 * i < kb) out[(g + 4 * i) * K + j] = 0.0;
+        wave_sync();
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+          if (live && g + 4 * i <= j) out[ar[i] * K + j] = lower ? iv[i] : cr[i] * iv[i];
+      }
+    } else {
+      // lane j forms column j by back substitution in registers (the loop over rows stays
+      // rolled; inv is written through a select chain)
+      double inv[K];
+      const int j = lane;
+      if (wave < 4) {
+        const double* Tw = Xsb + isp * 2 * K * K + (lower ? K * K : 0);
+#pragma unroll
+        for (int p = 0; p < K; ++p) inv[p] = p == j ? 1.0 : 0.0;
+        // row r of the table is read while row r+1 is being used (one LDS wait per row)
+        double trow[K];
+#pragma unroll
+        for (int p = 0; p < K; ++p) trow[p] = Tw[(K - 2) * K + p];
 #pragma unroll 1
-      for (int r = K - 2; r >= 0; --r) {
-        double tnext[K];
-        const int rn = r > 0 ? r - 1 : 0;
+        for (int r = K - 2; r >= 0; --r) {
+          double tnext[K];
+          const int rn = r > 0 ? r - 1 : 0;
 #pragma unroll
-        for (int p = 0; p < K; ++p) tnext[p] = Tw[rn * K + p];
-        // four interleaved partial sums shorten the dependent FMA chain
-        double v4[4] = {0.0, 0.0, 0.0, 0.0};
+          for (int p = 0; p < K; ++p) tnext[p] = Tw[rn * K + p];
+          // four interleaved partial sums shorten the dependent FMA chain
+          double v4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int p = 0; p < K; ++p) v4[p & 3] -= trow[p] * inv[p];
-        const double v = (v4[0] + v4[1]) + (v4[2] + v4[3]);
+          for (int p = 0; p < K; ++p) v4[p & 3] -= trow[p] * inv[p];
+          const double v = (v4[0] + v4[1]) + (v4[2] + v4[3]);
 #pragma unroll
-        for (int p = 0; p < K; ++p)
-          if (p == r && r < j) inv[p] = v;
+          for (int p = 0; p < K; ++p)
+            if (p == r && r < j) inv[p] = v;
 #pragma unroll
-        for (int p = 0; p < K; ++p) trow[p] = tnext[p];
+          for (int p = 0; p < K; ++p) trow[p] = tnext[p];
+        }
       }
-    }
-    __syncthreads();
-    if (wave < 4 && j < m) {
-      // scatter onto the block rows: Mx[aidx[r]][j] = c_r inv[r][j], My[aidx[r]][j] = inv[r][j]
-      double* out = (lower ? Ysb : Xsb) + isp * K * K;
-      const double* cs = cf + isp * K;
-      for (int qq = 0; qq < kb; ++qq) out[qq * K + j] = 0.0;
+      __syncthreads();
+      if (wave < 4 && j < m) {
+        double* out = (lower ? Ysb : Xsb) + isp * K * K;
+        const double* cs = cf + isp * K;
+        for (int qq = 0; qq < kb; ++qq) out[qq * K + j] = 0.0;
 #pragma unroll
-      for (int r = 0; r < K; ++r)
-        if (r <= j) out[aidx[r] * K + j] = lower ? inv[r] : cs[r] * inv[r];
+        for (int r = 0; r < K; ++r)
+          if (r <= j) out[aidx[r] * K + j] = lower ? inv[r] : cs[r] * inv[r];
+      }
     }
     __syncthreads();
     if (lrows) form_z<T, K, true>(G, n2, N, Np, b0, kb, m, Ysb, wsT, spmask, Yl);
