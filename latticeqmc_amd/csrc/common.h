@@ -197,8 +197,8 @@ struct FlipArgs {
   unsigned* bar;  // [nchains][4] barrier counters and XCC bits (wpc > 1), zero between launches
   int nchains;
   int bf3;        // delayed FP32 form: rank-k products as three split-BF16 matrix-core products
-  int yrows;      // delayed flip, k = 16, one spin per workgroup: the block's rows of G staged in LDS
-                  // for the Z tiles (flip_delayed_stage_rows; the LDS holds 16 x nmax more doubles)
+  int yrows;      // delayed flip, k = 16, one spin per workgroup: the block's rows of G and its Z tiles
+                  // kept in LDS (flip_delayed_stage_rows; flip_delayed_rows_shmem more bytes of LDS)
 };
 
 }  // namespace lqmc

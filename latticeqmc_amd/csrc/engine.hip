@@ -160,6 +160,7 @@ struct lqmc_ctx {
   int nmax = 0, lmax = 0;
   int flip_threads = 256;
   int flip_groups = 1;           // workgroups per chain of the delayed flip
+  int flip_yrows = 0;            // delayed flip keeps block rows and Z tiles in LDS (FlipArgs::yrows)
   unsigned* d_bar = nullptr;     // [nchains][4] their barrier counters and XCC bits
   unsigned* d_qrpub = nullptr;   // [4 nchains] hand-over counters of the on-chip QR (two per matrix)
   int cus = 0;                   // compute units of the device
@@ -720,7 +721,7 @@ static void run_ops(lqmc_ctx* x, const std::vector<Op>& ops, int measure, hipStr
       case OP_FLIP: {
         FlipArgs a{x->d_chains, x->d_state, op.t, measure, x->opt.rng, x->opt.trace, x->opt.delay,
                    x->flip_groups, x->d_bar, x->nchains, x->opt.precision == LQMC_BF16X3 ? 1 : 0,
-                   x->opt.delay > 1 ? flip_delayed_stage_rows(x->nmax, x->opt.delay, x->flip_groups) : 0};
+                   x->flip_yrows};
         launch_flip(fp32, x->opt.delay > 1, a, x->nchains, x->flip_threads, x->flip_shmem, s);
         break;
       }
@@ -791,7 +792,9 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
   }
   // The flip kernels keep their dynamic LDS within 64 KB per workgroup (the
   // delayed form's rank-k operands live in a per-chain global workspace), which
-  // bounds the supported delay for a given thread count.
+  // bounds the supported delay for a given thread count.  Only the delayed flip's
+  // staged block rows and Z tiles (16-site blocks, N <= 256) go beyond it, where the
+  // device's per-workgroup LDS allows (160 KB on gfx950).
   const int maxsh = 65536;
   const bool fp32 = o.precision != LQMC_FP64;
   const int tmax = o.delay > 1 ? flip_delayed_threads_limit(o.delay) : 1024;
@@ -816,6 +819,7 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
   // whole 16-row strips, or the rank-1 form, rows split evenly): all of them must be resident
   // at once, since they wait for each other, so the grid stays within one workgroup per
   // compute unit.
+  size_t lds_dev = 0;  // per-workgroup LDS of the device
   {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
@@ -824,6 +828,7 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
     }
     const int cus = prop.multiProcessorCount;
     x->cus = cus;
+    lds_dev = prop.sharedMemPerBlock;
     const int padded = 8 * ((n_chains + 7) / 8);
     const bool splittable = o.mode == LQMC_MODE_DQMC && (o.delay == 1 || o.delay == 16 || o.delay == 32);
     const int waves = x->flip_threads / 64;
@@ -860,11 +865,11 @@ int lqmc_create(lqmc_ctx** out, int device, int n_chains, const int32_t* n_sites
       }
     }
   }
-  // rows of each block staged in LDS for the Z tiles (16 x nmax doubles, nmax <= 256)
-  if (o.delay > 1) x->flip_shmem += flip_delayed_rows_shmem(x->nmax, o.delay, x->flip_groups);
-  if ((int)x->flip_shmem > maxsh) {
-    delete x;
-    return fail(LQMC_EINVAL, "flip kernel needs more than 64 KB of LDS; lower the delay");
+  // block rows and Z tiles of the delayed flip in LDS (k = 16, nmax <= 256), where they fit
+  if (o.delay > 1 && flip_delayed_stage_rows(x->nmax, o.delay, x->flip_groups) &&
+      x->flip_shmem + flip_delayed_rows_shmem(x->nmax, o.delay, x->flip_groups) <= lds_dev) {
+    x->flip_yrows = 1;
+    x->flip_shmem += flip_delayed_rows_shmem(x->nmax, o.delay, x->flip_groups);
   }
   if (o.mode == LQMC_MODE_DET) {
     x->det_shmem = det_sweep_shmem(x->nmax, x->lmax);

@@ -380,10 +380,11 @@ __device__ inline double quad_bcast(double v, int src) {
 // spmask: bit s set = the tiles of spin s are formed (a workgroup forms only the spins of the
 // strips it updates).
 // LROWS: the workgroup forms one spin's tiles and rows B of that spin are in LDS (Yl, leading
-// dimension N), read in place of G's rows.
-template <typename T, int K, bool LROWS = false>
+// dimension N), read in place of G's rows.  ZL: the tiles are stored to LDS (zl, tile t - tbase
+// of the workgroup's spin, same register order) instead of the workspace.
+template <typename T, int K, bool LROWS = false, bool ZL = false>
 __device__ void form_z(const T* G, size_t n2, int N, int Np, int b0, int kb, int m, const double* My, T* zws,
-                       int spmask = 3, const T* Yl = nullptr) {
+                       int spmask = 3, const T* Yl = nullptr, T* zl = nullptr) {
   using acc_t = typename MfmaOp<T>::acc_t;
   constexpr int JT = (K + 15) / 16;  // 16-wide tiles over the accepts
   constexpr int KS = K / 4;           // k-steps of one tile
@@ -425,9 +426,15 @@ __device__ void form_z(const T* G, size_t n2, int N, int Np, int b0, int kb, int
       acc_t z = acc_t{0, 0, 0, 0};
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) z = MfmaOp<T>::mma(xv[u][ks], yv[u][ks], z);
-      LQMC_GLOBAL T* zo = Zg + (size_t)t * 256 + lane;
+      if constexpr (ZL) {
+        T* zo = zl + (size_t)(t - tbase) * 256 + lane;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) zo[v * 64] = z[v];
+        for (int v = 0; v < 4; ++v) zo[v * 64] = z[v];
+      } else {
+        LQMC_GLOBAL T* zo = Zg + (size_t)t * 256 + lane;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) zo[v * 64] = z[v];
+      }
     }
   }
 }
@@ -439,10 +446,12 @@ __device__ void form_z(const T* G, size_t n2, int N, int Np, int b0, int kb, int
 // updated the tile holding columns B).
 // FULL: N is a multiple of 16 RU (every strip row and column-tile group lies inside G) and
 // K = 16 (one tile of accepts, m >= 1), so the tile loads and stores of the column loop need no
-// bounds checks; the products are the same.
-template <typename T, int K, int RU, bool BF3 = false, bool FULL = false>
+// bounds checks; the products are the same.  ZL: the Z tiles are read from LDS (zl, the layout
+// form_z<..., ZL> stores).
+template <typename T, int K, int RU, bool BF3 = false, bool FULL = false, bool ZL = false>
 __device__ void update_strip(T* G, size_t n2, int N, int Np, int b0, int kb, int m, const double* Mx,
-                             const T* zws, int rs, int cg = 0, int ng = 1, const double* pin = nullptr) {
+                             const T* zws, int rs, int cg = 0, int ng = 1, const double* pin = nullptr,
+                             const T* zl = nullptr) {
   static_assert(!FULL || K == 16, "full-tile form: one tile of accepts");
   using acc_t = typename MfmaOp<T>::acc_t;
   constexpr int JT = (K + 15) / 16;
@@ -490,9 +499,15 @@ __device__ void update_strip(T* G, size_t n2, int N, int Np, int b0, int kb, int
       for (int jt = 0; jt < JT; ++jt)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const size_t zi = ((size_t)((sp * nct + ct) * JT + jt) * 4 + v) * 64 + lane;
-          if constexpr (FULL) zv[u][jt][v] = Zg[zi];
-          else zv[u][jt][v] = ldg0(Zg, zi, ct < nct && jt < jtu);
+          if constexpr (ZL) {
+            const int zli = ((ct * JT + jt) * 4 + v) * 64 + lane;
+            if constexpr (FULL) zv[u][jt][v] = zl[zli];
+            else zv[u][jt][v] = ct < nct && jt < jtu ? zl[zli] : T(0);
+          } else {
+            const size_t zi = ((size_t)((sp * nct + ct) * JT + jt) * 4 + v) * 64 + lane;
+            if constexpr (FULL) zv[u][jt][v] = Zg[zi];
+            else zv[u][jt][v] = ldg0(Zg, zi, ct < nct && jt < jtu);
+          }
         }
     }
     if constexpr (BF3) {
@@ -694,7 +709,8 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
   double* Mxp = reinterpret_cast<double*>(m_sh + 4);  // [2][K][K] Mx of the previous block (W > 1)
   double* Pin = Mxp + 2 * K * K;  // [2][16][16] G0[rows, columns] of the deferred strips (K = 16)
   T* Yl = reinterpret_cast<T*>(Pin + (K == 16 ? 512 : 0));  // [16][N] rows B of the workgroup's spin (a.yrows)
-  int8_t* fl = reinterpret_cast<int8_t*>(Yl + (K == 16 && a.yrows ? 16 * N : 0));  // [N] the slice's field
+  T* Zl = Yl + (K == 16 && a.yrows ? 16 * N : 0);  // [Np / 16][256] Z tiles of the workgroup's spin (a.yrows)
+  int8_t* fl = reinterpret_cast<int8_t*>(Zl + (K == 16 && a.yrows ? 16 * Np : 0));  // [N] the slice's field
 
   const unsigned long long sweep = a.st[c].sweep;
   unsigned long long nacc = 0;
@@ -709,9 +725,11 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
   // whole strips and column-tile groups of 4 tiles: the strip updates need no bounds checks
   const bool full = K == 16 && N % 64 == 0;
   // one spin per workgroup (W > 1): rows B of that spin are staged in LDS by the helper waves
-  // (below) while wave 0 runs the serial phase, and the Z tiles read them there.  Rows B are final after
-  // the block's chain barrier and stay untouched until the next one (the strips holding them
-  // are deferred), so the staged rows equal what the Z tiles would read from G.
+  // (below) while wave 0 runs the serial phase, and the Z tiles read them there.  Rows B are
+  // final after the block's chain barrier and stay untouched until the next one (the strips
+  // holding them are deferred), so the staged rows equal what the Z tiles would read from G.
+  // With whole tiles (full) the Z tiles themselves stay in LDS too (Zl): the strips of the
+  // block and the deferred strips, which run before the next block's Z tiles, read them there.
   const bool lrows = K == 16 && a.yrows && W > 1 && (spmask == 1 || spmask == 2);
   for (int i = threadIdx.x; i < N; i += blockDim.x) fl[i] = fld[i];
   __syncthreads();
@@ -742,7 +760,8 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
         if ((rs % tw) / nw != w) continue;
         for (int g = 0; g < ngr; ++g, ++item)
           if (wave == (serial_busy ? helper_wave(item % nhw) : w0 + item % nwv))
-            if (full) update_strip<T, K, RUd, BF3, K == 16>(G, n2, N, Np, pb0, pkb, pm, Mxp, wsT, rs, g, ngr, Pin + sp * 256);
+            if (full && lrows) update_strip<T, K, RUd, BF3, K == 16, true>(G, n2, N, Np, pb0, pkb, pm, Mxp, wsT, rs, g, ngr, Pin + sp * 256, Zl);
+            else if (full) update_strip<T, K, RUd, BF3, K == 16>(G, n2, N, Np, pb0, pkb, pm, Mxp, wsT, rs, g, ngr, Pin + sp * 256);
             else update_strip<T, K, RUd, BF3>(G, n2, N, Np, pb0, pkb, pm, Mxp, wsT, rs, g, ngr, Pin + sp * 256);
       }
     } else {
@@ -1008,7 +1027,8 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
       }
     }
     __syncthreads();
-    if (lrows) form_z<T, K, true>(G, n2, N, Np, b0, kb, m, Ysb, wsT, spmask, Yl);
+    if (lrows && full) form_z<T, K, true, true>(G, n2, N, Np, b0, kb, m, Ysb, wsT, spmask, Yl, Zl);
+    else if (lrows) form_z<T, K, true>(G, n2, N, Np, b0, kb, m, Ysb, wsT, spmask, Yl);
     else form_z<T, K>(G, n2, N, Np, b0, kb, m, Ysb, wsT, spmask);
     __syncthreads();
     {
@@ -1018,7 +1038,8 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
       for (int rs = gw; rs < 2 * nct; rs += tw) {
         const int st = rs % nct;
         if (st < s0 || st > s1) {
-          if (full) update_strip<T, K, (K >= 16 ? 4 : 2), BF3, K == 16>(G, n2, N, Np, b0, kb, m, Xsb, wsT, rs);
+          if (full && lrows) update_strip<T, K, (K >= 16 ? 4 : 2), BF3, K == 16, true>(G, n2, N, Np, b0, kb, m, Xsb, wsT, rs, 0, 1, nullptr, Zl);
+          else if (full) update_strip<T, K, (K >= 16 ? 4 : 2), BF3, K == 16>(G, n2, N, Np, b0, kb, m, Xsb, wsT, rs);
           else update_strip<T, K, (K >= 16 ? 4 : 2), BF3>(G, n2, N, Np, b0, kb, m, Xsb, wsT, rs);
         }
       }
@@ -1422,12 +1443,12 @@ size_t flip_delayed_shmem(int nmax, int k, int threads, int fp32) {
          sizeof(int) * (2 * k + 4) + (size_t)nmax;
 }
 
-// Whether the delayed flip stages each block's rows in LDS (FlipArgs::yrows): 16-site blocks
-// on several workgroups per chain, N <= 256 (16 x nmax doubles, at most 32 KB, added to
-// flip_delayed_shmem).
+// Whether the delayed flip can stage each block's rows and Z tiles in LDS (FlipArgs::yrows):
+// 16-site blocks on several workgroups per chain, N <= 256.  The extra LDS is 16 x nmax
+// doubles of rows and 16 x Np doubles of Z tiles (at most 64 KB), added to flip_delayed_shmem.
 int flip_delayed_stage_rows(int nmax, int k, int groups) { return k == 16 && groups > 1 && nmax <= 256 ? 1 : 0; }
 size_t flip_delayed_rows_shmem(int nmax, int k, int groups) {
-  return flip_delayed_stage_rows(nmax, k, groups) ? sizeof(double) * 16 * (size_t)nmax : 0;
+  return flip_delayed_stage_rows(nmax, k, groups) ? sizeof(double) * 16 * ((size_t)nmax + ((nmax + 15) & ~15)) : 0;
 }
 
 // Bytes of the per-chain delayed-update workspace (the Z tiles of both spins, one set per
