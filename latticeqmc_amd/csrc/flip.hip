@@ -1051,11 +1051,22 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
       pm = m;
       for (int e = threadIdx.x; e < 2 * K * K; e += blockDim.x) Mxp[e] = Xsb[e];
       if constexpr (K == 16) {
-        // rows B x columns B of both spins (untouched until the deferred update)
-        for (int e = threadIdx.x; e < 512; e += blockDim.x) {
-          const int sp = e >> 8, i = (e >> 4) & 15, q = e & 15;
-          const bool ok = b0 + i < N && q < kb;
-          Pin[e] = ok ? (double)G[sp * n2 + (size_t)(b0 + i) * N + b0 + q] : 0.0;
+        // rows B x columns B of both spins (untouched until the deferred update); with the
+        // rows staged, the workgroup's spin is copied from LDS (the only spin its deferred
+        // strips use) and no global read sits before the next barrier
+        if (lrows) {
+          const int sp0 = spmask >> 1;
+          for (int e = threadIdx.x; e < 512; e += blockDim.x) {
+            const int sp = e >> 8, i = (e >> 4) & 15, q = e & 15;
+            const bool ok = sp == sp0 && b0 + i < N && q < kb;
+            Pin[e] = ok ? (double)Yl[i * N + b0 + q] : 0.0;
+          }
+        } else {
+          for (int e = threadIdx.x; e < 512; e += blockDim.x) {
+            const int sp = e >> 8, i = (e >> 4) & 15, q = e & 15;
+            const bool ok = b0 + i < N && q < kb;
+            Pin[e] = ok ? (double)G[sp * n2 + (size_t)(b0 + i) * N + b0 + q] : 0.0;
+          }
         }
       }
     }
