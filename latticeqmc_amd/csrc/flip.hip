@@ -733,7 +733,7 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
   const bool lrows = K == 16 && a.yrows && W > 1 && (spmask == 1 || spmask == 2);
   for (int i = threadIdx.x; i < N; i += blockDim.x) fl[i] = fld[i];
   __syncthreads();
-  bool pending = false;
+  bool pending = false, prepped = false;
   int pb0 = 0, pkb = 0, pm = 0;
   ChainSync cs = chain_sync_begin(bar, &a.st[c], W, m_sh + 1);
   // The deferred strips of the previous block (its own rows, after the chain barrier), with
@@ -781,13 +781,16 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
     if constexpr (K >= 16) {
       if (pending) chain_barrier(cs);
     }
-    if (threadIdx.x < kb) {
+    // the block's field entries and uniforms, unless wave 4 drew them during the previous
+    // block's inverses
+    if (!prepped && threadIdx.x < kb) {
       const int i = b0 + threadIdx.x;
       sb[threadIdx.x] = fl[i];
       bool und = false;
       ub[threadIdx.x] = draw_uniform(ch, &a.st[c], c, a.rng, sweep, a.t, i, und);
       if (und) underrun = true;
     }
+    prepped = false;
     __syncthreads();
     if (wave == 0) {
       // The k x k blocks of both spins live in wave 0's registers: lane
@@ -962,7 +965,18 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
           for (int i = 0; i < NI; ++i)
             if (4 * i < p) iv[i] = fma(-tr[i][p], ip, iv[i]);
         }
+      } else if (nw > 4 && wave == 4 && b0 + K < N) {
+        // wave 4, idle here: the next block's field entries (not changed by this block) and
+        // uniforms, read by wave 0 after this block's barriers
+        const int i = b0 + K + lane;
+        if (lane < min(K, N - b0 - K)) {
+          sb[lane] = fl[i];
+          bool und = false;
+          ub[lane] = draw_uniform(ch, &a.st[c], c, a.rng, sweep, a.t, i, und);
+          if (und) underrun = true;
+        }
       }
+      prepped = nw > 4 && b0 + K < N;
       __syncthreads();
       if (wave < 4) {
         const bool live = j < m;
