@@ -812,6 +812,11 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
       // the block's field entries and uniforms, lane jj holding site jj (picked by readlane)
       const int sb_l = sb[lane < kb ? lane : 0];
       const double ub_l = ub[lane < kb ? lane : 0];
+      // the chain's exp(+-2 lambda) held in VGPRs: as scalars the compiler keeps them with a
+      // 64-byte block of the chain record that it spills and reloads at every site
+      double e2pv = ch.e2p, e2mv = ch.e2m;
+      LQMC_OPAQUE(e2pv, "v");
+      LQMC_OPAQUE(e2mv, "v");
       // One site jj (hj = jj / R) with cj = G(current)[jj][q] of my spin on the lanes with
       // h == hj.  For R <= 8 the sites of one hj are unrolled, so cj is a register of col;
       // otherwise it is picked by a select chain.
@@ -820,7 +825,7 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
         const double gu = readlane_f64(cj, hj * K + jj);
         const double gd = readlane_f64(cj, K * H + hj * K + jj);
         const int s = __builtin_amdgcn_readlane(sb_l, jj);
-        const double e2p = ch.e2p, e2m = ch.e2m;
+        const double e2p = e2pv, e2m = e2mv;
         const double ep = s > 0 ? e2p : e2m;
         const double em = s > 0 ? e2m : e2p;
         const double d_up = 1.0 + (1.0 - gu) * (ep - 1.0);
@@ -828,9 +833,9 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
         const double ratio = d_up * d_dn;
         const bool acc = readlane_f64(ub_l, jj) <= ratio;
         if (!isfinite(ratio)) bad = true;
-        if (a.trace && lane == 0 && w == 0) {
-          ch.tr_ratio[(size_t)a.t * N + b0 + jj] = ratio;
-          ch.tr_acc[(size_t)a.t * N + b0 + jj] = acc ? 1 : 0;
+        if (a.trace && lane == 0) {  // to LDS (red is free until the epilogue); written out below
+          red[jj] = ratio;
+          red[K + jj] = acc ? 1.0 : 0.0;
         }
         // The exchanges and the coefficient do not depend on the decision: they are
         // issued before it, so their latencies overlap.  My spin's coefficient is
@@ -902,6 +907,10 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
       if (pending) deferred_strips(true);
     }
     __syncthreads();
+    if (a.trace && w == 0 && threadIdx.x < kb) {
+      ch.tr_ratio[(size_t)a.t * N + b0 + threadIdx.x] = red[threadIdx.x];
+      ch.tr_acc[(size_t)a.t * N + b0 + threadIdx.x] = red[K + threadIdx.x] != 0.0 ? 1 : 0;
+    }
     pending = false;
     const int m = m_sh[0];
     if (m == 0) continue;
