@@ -1,6 +1,7 @@
 """Build liblqmc_hip.so (gfx950) in-tree with hipcc.
 
-    python -m latticeqmc_amd.build
+    python -m latticeqmc_amd.build            (liblqmc_hip.so)
+    python -m latticeqmc_amd.build --probe    (liblqmc_hip_probe.so, flip phase clocks)
 
 Every HIP source is compiled with packed-FP32 instructions disabled
 (-Xclang -target-feature -Xclang -packed-fp32-ops) and linked into one shared
@@ -40,13 +41,21 @@ def flags():
 EXTRA = {"gemm.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
-def build(verbose=False):
-    os.makedirs(OBJ, exist_ok=True)
+PROBE_LIB = os.path.join(HERE, "liblqmc_hip_probe.so")
+
+
+def build(verbose=False, probe=False):
+    """Compile and link the library.  probe=True builds liblqmc_hip_probe.so instead: the same
+    sources with the delayed flip's phase clocks compiled in (-DLQMC_FLIP_PROBE), for
+    latticeqmc_amd.flip_phases; the product library never carries them."""
+    obj_dir = os.path.join(OBJ, "probe") if probe else OBJ
+    os.makedirs(obj_dir, exist_ok=True)
     cc = hipcc()
+    extra_all = ["-DLQMC_FLIP_PROBE"] if probe else []
 
     def compile_one(src):
-        obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-        cmd = [cc, *flags(), *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
+        obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
+        cmd = [cc, *flags(), *extra_all, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")
@@ -56,14 +65,15 @@ def build(verbose=False):
 
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         objs = list(ex.map(compile_one, SOURCES))
-    tmp = LIB + ".tmp"
+    out = PROBE_LIB if probe else LIB
+    tmp = out + ".tmp"
     cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, probe="--probe" in sys.argv[1:]))

@@ -349,6 +349,19 @@ __device__ inline double quad_bcast(double v, int src) {
   }
 }
 
+// Phase clocks of the delayed flip (built only with -DLQMC_FLIP_PROBE, as the separate
+// liblqmc_hip_probe.so of latticeqmc_amd.flip_phases): thread 0 of each workgroup sums the
+// wall-clock ticks (100 MHz) of each phase of its blocks and, with tracing on, writes them over
+// the slice's trace ratios at t N + 16 w + p (p < 14; 14: blocks, 15: total) when the launch
+// ends; the per-site trace is not written.  Phases: 0 chain barrier, 1 uniforms and sync,
+// 2 wave 0's diagonal-block loads, 3 sites, 4 wait for the other waves, 12 tables,
+// 13 inverses, 5 scatter, 6 Z tiles, 7 strips and copies, 8 tail.
+#ifdef LQMC_FLIP_PROBE
+#define LQMC_PROBE(p) probe_mark(p)
+#else
+#define LQMC_PROBE(p) ((void)0)
+#endif
+
 // Per block of k sites (k a multiple of 4):
 //  1. the block's field entries and uniforms go to LDS;
 //  2. wave 0 runs the k attempts on the k x k diagonal blocks of both spins, held in
@@ -735,6 +748,17 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
   __syncthreads();
   bool pending = false, prepped = false;
   int pb0 = 0, pkb = 0, pm = 0;
+#ifdef LQMC_FLIP_PROBE
+  unsigned long long ph[16] = {0};
+  unsigned long long tprev = wall_clock64();
+  const unsigned long long tstart = tprev;
+  int nblk = 0;
+  auto probe_mark = [&](int p) {
+    const unsigned long long tt = wall_clock64();
+    ph[p] += tt - tprev;
+    tprev = tt;
+  };
+#endif
   ChainSync cs = chain_sync_begin(bar, &a.st[c], W, m_sh + 1);
   // The deferred strips of the previous block (its own rows, after the chain barrier), with
   // that block's Mx kept in Mxp and its Z tiles still in the workspace.  Each is applied by
@@ -781,6 +805,10 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
     if constexpr (K >= 16) {
       if (pending) chain_barrier(cs);
     }
+    LQMC_PROBE(0);
+#ifdef LQMC_FLIP_PROBE
+    ++nblk;
+#endif
     // the block's field entries and uniforms, unless wave 4 drew them during the previous
     // block's inverses
     if (!prepped && threadIdx.x < kb) {
@@ -792,6 +820,7 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
     }
     prepped = false;
     __syncthreads();
+    LQMC_PROBE(1);
     if (wave == 0) {
       // The k x k blocks of both spins live in wave 0's registers: lane
       // (sp, h, q) = (lane / (K H), lane / K % H, lane % K) holds rows h R .. h R + R-1
@@ -812,6 +841,10 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
       // the block's field entries and uniforms, lane jj holding site jj (picked by readlane)
       const int sb_l = sb[lane < kb ? lane : 0];
       const double ub_l = ub[lane < kb ? lane : 0];
+#ifdef LQMC_FLIP_PROBE
+      __builtin_amdgcn_s_waitcnt(0);
+      LQMC_PROBE(2);
+#endif
       // the chain's exp(+-2 lambda) held in VGPRs: as scalars the compiler keeps them with a
       // 64-byte block of the chain record that it spills and reloads at every site
       double e2pv = ch.e2p, e2mv = ch.e2m;
@@ -833,7 +866,11 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
         const double ratio = d_up * d_dn;
         const bool acc = readlane_f64(ub_l, jj) <= ratio;
         if (!isfinite(ratio)) bad = true;
+#ifdef LQMC_FLIP_PROBE
+        if (false) {
+#else
         if (a.trace && lane == 0) {  // to LDS (red is free until the epilogue); written out below
+#endif
           red[jj] = ratio;
           red[K + jj] = acc ? 1.0 : 0.0;
         }
@@ -889,6 +926,7 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
         }
       }
       if (lane == 0) m_sh[0] = m;
+      LQMC_PROBE(3);
     } else if (K >= 16) {
       if (lrows) {
         const LQMC_GLOBAL T* Gs = gptr(G) + (spmask >> 1) * n2 + (size_t)b0 * N;
@@ -907,12 +945,17 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
       if (pending) deferred_strips(true);
     }
     __syncthreads();
+#ifdef LQMC_FLIP_PROBE
+    if (false) {
+#else
     if (a.trace && w == 0 && threadIdx.x < kb) {
+#endif
       ch.tr_ratio[(size_t)a.t * N + b0 + threadIdx.x] = red[threadIdx.x];
       ch.tr_acc[(size_t)a.t * N + b0 + threadIdx.x] = red[K + threadIdx.x] != 0.0 ? 1 : 0;
     }
     pending = false;
     const int m = m_sh[0];
+    LQMC_PROBE(4);
     if (m == 0) continue;
     nacc += m;
     // The coefficients of the two unit upper-triangular systems per spin, in accept
@@ -943,6 +986,7 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
       }
     }
     __syncthreads();
+    LQMC_PROBE(12);
     // unit upper-triangular inverses, one per wave: waves 0/1 spin 0 (I+UC)^-1 / (I-L)^-1,
     // waves 2/3 spin 1, then scattered onto the block rows:
     // Mx[aidx[r]][j] = c_r inv[r][j], My[aidx[r]][j] = inv[r][j] (zero on rejected sites).
@@ -987,6 +1031,7 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
       }
       prepped = nw > 4 && b0 + K < N;
       __syncthreads();
+      LQMC_PROBE(13);
       if (wave < 4) {
         const bool live = j < m;
         double* out = (lower ? Ysb : Xsb) + isp * K * K;
@@ -1050,10 +1095,12 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
       }
     }
     __syncthreads();
+    LQMC_PROBE(5);
     if (lrows && full) form_z<T, K, true, true>(G, n2, N, Np, b0, kb, m, Ysb, wsT, spmask, Yl, Zl);
     else if (lrows) form_z<T, K, true>(G, n2, N, Np, b0, kb, m, Ysb, wsT, spmask, Yl);
     else form_z<T, K>(G, n2, N, Np, b0, kb, m, Ysb, wsT, spmask);
     __syncthreads();
+    LQMC_PROBE(6);
     {
       // W > 1: the strips of this block's rows wait for the next barrier (see above)
       const bool defer = K >= 16 && W > 1;
@@ -1094,6 +1141,7 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
       }
     }
     __syncthreads();
+    LQMC_PROBE(7);
   }
   if constexpr (K >= 16) {
     if (pending) {
@@ -1109,6 +1157,14 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
     for (int i = threadIdx.x; i < N; i += blockDim.x) fld[i] = fl[i];
     slice_epilogue<T>(a, ch, c, l, G, nacc, underrun, bad, red);
   }
+#ifdef LQMC_FLIP_PROBE
+  LQMC_PROBE(8);
+  if (a.trace && threadIdx.x == 0 && 16 * (w + 1) <= N) {
+    for (int p = 0; p < 14; ++p) ch.tr_ratio[(size_t)a.t * N + w * 16 + p] = (double)ph[p];
+    ch.tr_ratio[(size_t)a.t * N + w * 16 + 14] = (double)nblk;
+    ch.tr_ratio[(size_t)a.t * N + w * 16 + 15] = (double)(wall_clock64() - tstart);
+  }
+#endif
   if (W > 1) chain_exit(cs);
 }
 
