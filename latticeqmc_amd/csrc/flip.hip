@@ -401,7 +401,8 @@ __device__ void form_z(const T* G, size_t n2, int N, int Np, int b0, int kb, int
   using acc_t = typename MfmaOp<T>::acc_t;
   constexpr int JT = (K + 15) / 16;  // 16-wide tiles over the accepts
   constexpr int KS = K / 4;           // k-steps of one tile
-  constexpr int TB = K >= 32 ? 2 : 4; // tiles per wavefront whose loads are issued together
+  // tiles per wavefront whose loads are issued together (ZL: one spin, 16 tiles over 8 waves)
+  constexpr int TB = K >= 32 || ZL ? 2 : 4;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int nct = Np >> 4;
