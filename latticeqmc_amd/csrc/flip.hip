@@ -870,7 +870,11 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
 #ifdef LQMC_FLIP_PROBE
         if (false) {
 #else
-        if (a.trace && lane == 0) {  // to LDS (red is free until the epilogue); written out below
+        // to LDS (red is free until the epilogue), written out below; the lane test is recomputed
+        // from a VGPR copy (kept as a lane mask it is spilled and reloaded at every site)
+        int lv = lane;
+        LQMC_OPAQUE(lv, "v");
+        if (a.trace && lv == 0) {
 #endif
           red[jj] = ratio;
           red[K + jj] = acc ? 1.0 : 0.0;
@@ -885,8 +889,12 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
         double xp[R];
 #pragma unroll
         for (int i = 0; i < R; ++i) xp[i] = __shfl(col[i], src, 64);
-        const double num = sp == 0 ? ep - 1.0 : em - 1.0;
-        const double cfs = num / (sp == 0 ? d_up : d_dn);
+        // the spin test is recomputed from a VGPR copy at every site (kept as a lane mask it is
+        // spilled and reloaded)
+        int spv = sp;
+        LQMC_OPAQUE(spv, "v");
+        const double num = spv == 0 ? ep - 1.0 : em - 1.0;
+        const double cfs = num / (spv == 0 ? d_up : d_dn);
         if (acc) {
           if (q == jj) yq = yq + 1.0;
           // G <- G - c x y^T on the k x k block.  Entries of rows >= kb and of lanes past
