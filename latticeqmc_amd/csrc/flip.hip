@@ -724,7 +724,10 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
   double* Pin = Mxp + 2 * K * K;  // [2][16][16] G0[rows, columns] of the deferred strips (K = 16)
   T* Yl = reinterpret_cast<T*>(Pin + (K == 16 ? 512 : 0));  // [16][N] rows B of the workgroup's spin (a.yrows)
   T* Zl = Yl + (K == 16 && a.yrows ? 16 * N : 0);  // [Np / 16][256] Z tiles of the workgroup's spin (a.yrows)
-  int8_t* fl = reinterpret_cast<int8_t*>(Zl + (K == 16 && a.yrows ? 16 * Np : 0));  // [N] the slice's field
+  // [16][16] G0[rows of the next block, columns B] and the count of waves done with that strip
+  double* PinN = reinterpret_cast<double*>(Zl + (K == 16 && a.yrows ? 16 * Np : 0));
+  unsigned* ecnt = reinterpret_cast<unsigned*>(PinN + (K == 16 && a.yrows ? 256 : 0));
+  int8_t* fl = reinterpret_cast<int8_t*>(ecnt + (K == 16 && a.yrows ? 4 : 0));  // [N] the slice's field
 
   const unsigned long long sweep = a.st[c].sweep;
   unsigned long long nacc = 0;
@@ -745,6 +748,15 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
   // With whole tiles (full) the Z tiles themselves stay in LDS too (Zl): the strips of the
   // block and the deferred strips, which run before the next block's Z tiles, read them there.
   const bool lrows = K == 16 && a.yrows && W > 1 && (spmask == 1 || spmask == 2);
+  // Early arrival (one spin per workgroup, whole tiles, eight waves, one strip per wave of the
+  // chain, L2-local barriers): the only rows another workgroup reads in the next block are that
+  // block's rows, so a workgroup arrives at the chain barrier as soon as those rows are final --
+  // at once after its Z tiles if it does not hold their strip, else after the two waves idle in
+  // its strip phase (the ones whose strips are this block's and the next block's rows) have
+  // updated that strip first and their stores are in L2.  The barrier at the next block's start
+  // then only waits.
+  const bool early = lrows && full && nw == 8 && tw == 2 * nct;
+  bool arrived = false;
   for (int i = threadIdx.x; i < N; i += blockDim.x) fl[i] = fld[i];
   __syncthreads();
   bool pending = false, prepped = false;
@@ -804,7 +816,20 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
   for (int b0 = 0; b0 < N; b0 += K) {
     const int kb = min(K, N - b0);
     if constexpr (K >= 16) {
-      if (pending) chain_barrier(cs);
+      if (pending) {
+        if (arrived) {
+          // arrival made during the previous block: wait, then acquire
+          const unsigned target = ++cs.n * cs.W;
+          if (threadIdx.x == 0) {
+            chain_wait(cs, target);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+          }
+          __syncthreads();
+        } else {
+          chain_barrier(cs);
+        }
+      }
+      arrived = false;
     }
     LQMC_PROBE(0);
 #ifdef LQMC_FLIP_PROBE
@@ -1027,6 +1052,14 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
           for (int i = 0; i < NI; ++i)
             if (4 * i < p) iv[i] = fma(-tr[i][p], ip, iv[i]);
         }
+      } else if (early && wave == 5 && b0 + K < N && cs.decided && cs.local) {
+        // the next block's strip of this spin, if this workgroup holds it: G0[next rows, B]
+        const int rsn = (spmask >> 1) * nct + (b0 >> 4) + 1;
+        if ((rsn % tw) / nw == w) {
+          for (int e = lane; e < 256; e += 64)
+            PinN[e] = (double)G[(spmask >> 1) * n2 + (size_t)(b0 + K + (e >> 4)) * N + b0 + (e & 15)];
+          if (lane == 0) ecnt[0] = 0u;
+        }
       } else if (nw > 4 && wave == 4 && b0 + K < N) {
         // wave 4, idle here: the next block's field entries (not changed by this block) and
         // uniforms, read by wave 0 after this block's barriers
@@ -1114,9 +1147,31 @@ __global__ __launch_bounds__(flip_delayed_max_threads(K)) void k_flip_delayed(Fl
       // W > 1: the strips of this block's rows wait for the next barrier (see above)
       const bool defer = K >= 16 && W > 1;
       const int s0 = defer ? b0 >> 4 : nct, s1 = defer ? (b0 + kb - 1) >> 4 : -1;
+      int rs_skip = -1;  // the next block's strip, when updated first (below)
+      if (K == 16 && early && b0 + K < N && cs.decided && cs.local) {
+        const int rsn = (spmask >> 1) * nct + (b0 >> 4) + 1, rsb = rsn - 1;
+        if ((rsn % tw) / nw != w) {
+          if (threadIdx.x == 0) atomicAdd(cs.bar, 1u);  // no row of the next block here
+        } else {
+          // its column-tile groups split between the wave of the next block's strip and, when
+          // this workgroup holds it, the wave of this block's (deferred) strip; P from PinN
+          const int vn = rsn % nw, vb = (rsb % tw) / nw == w ? rsb % nw : vn;
+          const int ngr = nct / 4, half = vb == vn ? ngr : ngr / 2;
+          if (wave == vn || wave == vb) {
+            const int g0 = wave == vn ? 0 : half, g1 = wave == vn ? half : ngr;
+            for (int g = g0; g < g1; ++g)
+              update_strip<T, K, 4, BF3, K == 16, true>(G, n2, N, Np, b0, kb, m, Xsb, wsT, rsn, g, ngr, PinN, Zl);
+            __builtin_amdgcn_s_waitcnt(0);  // this wave's stores of the strip are in the shared L2
+            wave_sync();
+            if (lane == 0 && atomicAdd(ecnt, 1u) == (vb == vn ? 0u : 1u)) atomicAdd(cs.bar, 1u);
+          }
+          rs_skip = rsn;
+        }
+        arrived = true;
+      }
       for (int rs = gw; rs < 2 * nct; rs += tw) {
         const int st = rs % nct;
-        if (st < s0 || st > s1) {
+        if ((st < s0 || st > s1) && rs != rs_skip) {
           if (full && lrows) update_strip<T, K, (K >= 16 ? 4 : 2), BF3, K == 16, true>(G, n2, N, Np, b0, kb, m, Xsb, wsT, rs, 0, 1, nullptr, Zl);
           else if (full) update_strip<T, K, (K >= 16 ? 4 : 2), BF3, K == 16>(G, n2, N, Np, b0, kb, m, Xsb, wsT, rs);
           else update_strip<T, K, (K >= 16 ? 4 : 2), BF3>(G, n2, N, Np, b0, kb, m, Xsb, wsT, rs);
@@ -1547,7 +1602,8 @@ size_t flip_delayed_shmem(int nmax, int k, int threads, int fp32) {
 // doubles of rows and 16 x Np doubles of Z tiles (at most 64 KB), added to flip_delayed_shmem.
 int flip_delayed_stage_rows(int nmax, int k, int groups) { return k == 16 && groups > 1 && nmax <= 256 ? 1 : 0; }
 size_t flip_delayed_rows_shmem(int nmax, int k, int groups) {
-  return flip_delayed_stage_rows(nmax, k, groups) ? sizeof(double) * 16 * ((size_t)nmax + ((nmax + 15) & ~15)) : 0;
+  return flip_delayed_stage_rows(nmax, k, groups)
+             ? sizeof(double) * (16 * ((size_t)nmax + ((nmax + 15) & ~15)) + 256) + 4 * sizeof(int) : 0;
 }
 
 // Bytes of the per-chain delayed-update workspace (the Z tiles of both spins, one set per
