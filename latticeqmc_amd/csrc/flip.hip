@@ -1599,7 +1599,8 @@ size_t flip_delayed_shmem(int nmax, int k, int threads, int fp32) {
 
 // Whether the delayed flip can stage each block's rows and Z tiles in LDS (FlipArgs::yrows):
 // 16-site blocks on several workgroups per chain, N <= 256.  The extra LDS is 16 x nmax
-// doubles of rows and 16 x Np doubles of Z tiles (at most 64 KB), added to flip_delayed_shmem.
+// doubles of rows, 16 x Np doubles of Z tiles and a 16 x 16 copy of the next block's G0[rows, B]
+// with its counter (at most 66 KB), added to flip_delayed_shmem.
 int flip_delayed_stage_rows(int nmax, int k, int groups) { return k == 16 && groups > 1 && nmax <= 256 ? 1 : 0; }
 size_t flip_delayed_rows_shmem(int nmax, int k, int groups) {
   return flip_delayed_stage_rows(nmax, k, groups)
